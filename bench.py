@@ -1,0 +1,203 @@
+"""Benchmark of the loop_tool hip backend on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nest softmax|matmul|conv2d|conv2d_bf16|maxpool]
+
+Default workload: FP32 row softmax over an 8192 x 8192 tensor (max-reduce,
+subtract, exp, sum-reduce, divide over the columns), written with the lazy
+frontend and compiled by the "hip" backend.  A step is one run of the
+compiled nest over the whole tensor with the input already resident in HBM.
+`value` is algorithmic HBM bytes (one read + one write of the tensor) per
+second over the timed steps.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E datasheet
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f32": 157.3}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nest", default="softmax")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+def workload(lt, name):
+    from loop_tool_amd import programs as P
+    if name == "softmax":
+        R, C = 8192, 8192
+        out, shapes = P.softmax(lt, R, C)
+        gen = lambda: [P.normal(3, shapes[0], 3.0)]
+        cfg = dict(workload="row softmax fp32 (max, subtract, exp, sum, divide over C)",
+                   rows=R, cols=C)
+        return out, shapes, gen, cfg, "hbm", "f32", "hip"
+    if name == "matmul":
+        out, shapes = P.matmul(lt, 128, 128, 128)
+        gen = lambda: [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
+        return out, shapes, gen, dict(workload="fp32 matmul M=N=K=128"), "mfma", "f32", "hip"
+    if name in ("conv2d", "conv2d_bf16"):
+        N, Ci, Co, H = 32, 128, 128, 56
+        out, shapes = P.conv2d(lt, N, Ci, Co, H, H)
+        std = 1.0 / np.sqrt(Ci * 9)
+        gen = lambda: [P.normal(4, shapes[0], std), P.normal(5, shapes[1], std)]
+        bf = name == "conv2d_bf16"
+        cfg = dict(workload="conv2d 3x3 pad 1 stride 1 (pad nest + window sum)",
+                   N=N, C_in=Ci, C_out=Co, H=H, W=H, operands="bf16" if bf else "fp32")
+        return out, shapes, gen, cfg, "mfma", "bf16" if bf else "f32", (
+            "hip_bf16" if bf else "hip")
+    if name == "maxpool":
+        out, shapes = P.maxpool2d(lt, 32, 128, 112, 112)
+        gen = lambda: [P.normal(6, shapes[0])]
+        return out, shapes, gen, dict(workload="2x2 max-pool stride 2", N=32, C=128,
+                                      H=112, W=112), "hbm", "f32", "hip"
+    raise SystemExit("unknown nest " + name)
+
+
+def cpu_baseline(name, inputs_np, shapes, seconds, bytes_per_step, flops_per_step):
+    """Time the reference C backend (oracle/_ref, else its restatement) on
+    host cores over a bounded sample of the same workload."""
+    from loop_tool_amd import capi, programs as P
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libloop_tool_ref.so")
+    kind = "reference"
+    if os.path.exists(ref_path):
+        L = capi.Library(ref_path)
+    else:
+        import loop_tool_amd as lt_mod
+        lt_mod.load_lib(os.path.join(ROOT, "oracle", "liblt_oracle.so"))
+        L, kind = lt_mod.lib, "port"
+    # sample: a slice of the rows for softmax, the full nest otherwise
+    if name == "softmax":
+        rows = 512
+        out, sh = P.softmax(L, rows, shapes[0][1])
+        ins = [inputs_np[0][:rows]]
+        scale = rows / shapes[0][0]
+        sample = "%d of %d rows" % (rows, shapes[0][0])
+    else:
+        out, sh = workload(L, name)[:2]
+        ins = inputs_np
+        scale = 1.0
+        sample = "full workload"
+    lt = out.loop_tree
+    cc = L.compile(lt, "cpp")
+    y = np.zeros(lt.io_numel()[len(ins)], np.float32)
+    ins = [np.ascontiguousarray(a, np.float32) for a in ins]
+    cc.run(ins + [y])  # warm-up (page-in)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        cc.run(ins + [y])
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or reps >= 1000:
+            break
+    per_rep = el / reps
+    if flops_per_step and not bytes_per_step:
+        value, unit = flops_per_step * scale / per_rep / 1e12, "TFLOP/s"
+    else:
+        value, unit = bytes_per_step * scale / per_rep / 1e9, "GB/s"
+    return dict(value=round(value, 4), unit=unit, cores=1, kind=kind,
+                sample="%s, %d reps, %.1f s, single thread (reference C backend, "
+                       "cc -O3 -std=c99)" % (sample, reps, el))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import loop_tool_amd as lt_mod
+    from loop_tool_amd import hip
+
+    L = lt_mod.lib
+    hip.init()
+    out, shapes, gen, cfg, bound, dtype, backend = workload(L, args.nest)
+    lt = out.loop_tree
+    cc = L.compile(lt, backend)
+    inputs_np = gen()
+    n_out = lt.io_numel()[len(inputs_np)]
+    d_in = [hip.DeviceArray.from_numpy(a) for a in inputs_np]
+    d_out = hip.DeviceArray(n_out)
+    io = d_in + [d_out]
+    st = hip.stages(cc)
+    bytes_step = sum(s["bytes"] for s in st)
+    flops_step = sum(s["flops"] for s in st)
+
+    # correctness spot check against float64 on the device result
+    from loop_tool_amd import programs as P
+    cc.run([b.ptr for b in io])
+    hip.synchronize()
+    y = d_out.download()
+    if args.nest == "softmax":
+        ref = P.reference_output("softmax", [inputs_np[0][:64]]).ravel()
+        err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
+    else:
+        err = None
+
+    for _ in range(args.warmup):
+        cc.run([b.ptr for b in io], sync=False)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cc.run([b.ptr for b in io], sync=False)
+    hip.synchronize()
+    el = time.perf_counter() - t0
+    ms = el / args.steps * 1e3
+
+    # dominant kernel: launch duration from HIP events on its own stream
+    kern_ms = hip.time_ms(cc, io, warmup=3, iters=max(args.steps, 20))
+    if bound == "hbm":
+        value = bytes_step * args.steps / el / 1e9
+        unit = "GB/s"
+        achieved = bytes_step / (kern_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+        metric = "HBM GB/s (algorithmic bytes / time), hip backend"
+    else:
+        value = flops_step * args.steps / el / 1e12
+        unit = "TFLOP/s"
+        achieved = flops_step / (kern_ms * 1e-3) / 1e12
+        peak = MFMA_PEAK_TFLOPS[dtype]
+        roof = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4), traffic=None)
+        metric = "TFLOP/s (algorithmic flops / time), hip backend"
+
+    line = {
+        "metric": metric,
+        "value": round(value, 3),
+        "unit": unit,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (SplitMix64-seeded inputs)",
+        "config": dict(cfg, kernels=[s["kernel"] + ":" + s["kind"] for s in st],
+                       algorithmic_bytes=bytes_step, algorithmic_flops=flops_step,
+                       kernel_ms=round(kern_ms, 5), max_err_vs_f64=err),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.nest, inputs_np, shapes, args.cpu_seconds,
+                                            bytes_step if bound == "hbm" else 0,
+                                            flops_step if bound != "hbm" else 0)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

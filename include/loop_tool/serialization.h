@@ -1,0 +1,17 @@
+// Text serialization of an IR (nodes, constraints, orders, annotations,
+// reuse flags, inputs/outputs) so that a tuned schedule can be stored and
+// reloaded.  Same entry points as include/loop_tool/serialization.h of the
+// reference; the text format is this project's own (one record per line,
+// expressions in prefix form) and round-trips losslessly.
+#pragma once
+
+#include <string>
+
+#include "ir.h"
+
+namespace loop_tool {
+
+std::string serialize(const IR &ir);
+IR deserialize(const std::string &str);
+
+}  // namespace loop_tool

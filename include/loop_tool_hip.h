@@ -1,0 +1,51 @@
+/*
+ * loop_tool_hip.h -- C ABI extensions of the "hip" backend's Compiled objects
+ * (handles from loop_tool_c.h compiled with backend "hip" or "hip_bf16").
+ *
+ * No counterpart in the reference: its Compiled::run(memory, sync)
+ * (include/loop_tool/backend.h:18-22) is the only launch entry point.  These
+ * calls let callers that keep tensors resident in HBM launch a compiled nest
+ * on their own stream with their own scratch (serving, graph capture,
+ * benchmarks) and read the plan's per-stage accounting.
+ */
+#ifndef LOOP_TOOL_HIP_H_
+#define LOOP_TOOL_HIP_H_
+
+#include <stdint.h>
+
+#include "loop_tool_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#pragma GCC visibility push(default)
+
+/* 1 if the Compiled was produced by the hip backend */
+int lth_is_hip(const ltc_compiled *cc);
+/* the Compiled's own stream (created on first use) */
+int lth_compiled_stream(const ltc_compiled *cc, void **stream);
+/* number of plan stages (kernel launches per run) */
+int lth_num_stages(const ltc_compiled *cc);
+/* kernel name, kind, algorithmic HBM bytes and flops of stage i */
+int lth_stage_info(const ltc_compiled *cc, int i, char *name, int name_cap, char *kind,
+                   int kind_cap, int64_t *bytes, int64_t *flops);
+/* number and sizes (float elements) of the scratch buffers a launch needs */
+int lth_scratch_sizes(const ltc_compiled *cc, int64_t *out, int cap);
+/* launch on `stream` with device buffers io (inputs then outputs) and
+ * scratch; no allocation, copy or synchronisation */
+int lth_run_on(const ltc_compiled *cc, void *const *io, int n_io, void *const *scratch,
+               int n_scratch, void *stream);
+/* time `iters` back-to-back launches on the Compiled's stream with HIP
+ * events after `warmup` untimed ones; io are device buffers.  Writes the
+ * mean milliseconds per launch of the whole plan to *ms */
+int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int iters,
+             float *ms);
+
+#pragma GCC visibility pop
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOP_TOOL_HIP_H_ */

@@ -1,0 +1,120 @@
+// hip backend extensions of the C ABI (see include/loop_tool_hip.h).
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "hip/hip_backend.h"
+#include "loop_tool_hip.h"
+#include "lt_hip.h"
+
+struct ltc_compiled {
+  std::shared_ptr<loop_tool::Compiled> cc;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+const loop_tool::hip::HipCompiled *as_hip(const ltc_compiled *cc) {
+  auto *h = dynamic_cast<const loop_tool::hip::HipCompiled *>(cc->cc.get());
+  if (!h) throw std::runtime_error("not a hip Compiled");
+  return h;
+}
+
+template <typename F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    g_err = e.what();
+  }
+  return -1;
+}
+
+int copy_str(const std::string &s, char *buf, int cap) {
+  if (buf && cap > 0) {
+    size_t n = std::min(s.size(), (size_t)cap - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  }
+  return (int)s.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lth_last_error(void) { return g_err.c_str(); }
+
+int lth_is_hip(const ltc_compiled *cc) {
+  return dynamic_cast<const loop_tool::hip::HipCompiled *>(cc->cc.get()) ? 1 : 0;
+}
+
+int lth_compiled_stream(const ltc_compiled *cc, void **stream) {
+  return guarded([&] {
+    *stream = as_hip(cc)->stream();
+    return 0;
+  });
+}
+
+int lth_num_stages(const ltc_compiled *cc) {
+  return guarded([&] { return (int)as_hip(cc)->plan().stages.size(); });
+}
+
+int lth_stage_info(const ltc_compiled *cc, int i, char *name, int name_cap, char *kind,
+                   int kind_cap, int64_t *bytes, int64_t *flops) {
+  return guarded([&] {
+    const auto &s = as_hip(cc)->plan().stages.at(i);
+    copy_str(s.kernel, name, name_cap);
+    copy_str(s.kind, kind, kind_cap);
+    *bytes = s.bytes;
+    *flops = s.flops;
+    return 0;
+  });
+}
+
+int lth_scratch_sizes(const ltc_compiled *cc, int64_t *out, int cap) {
+  return guarded([&] {
+    auto s = as_hip(cc)->scratch_sizes();
+    for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = s[i];
+    return (int)s.size();
+  });
+}
+
+int lth_run_on(const ltc_compiled *cc, void *const *io, int n_io, void *const *scratch,
+               int n_scratch, void *stream) {
+  return guarded([&] {
+    as_hip(cc)->run_on(std::vector<void *>(io, io + n_io),
+                       std::vector<void *>(scratch, scratch + n_scratch), stream);
+    return 0;
+  });
+}
+
+int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int iters,
+             float *ms) {
+  return guarded([&] {
+    auto *h = as_hip(cc);
+    std::vector<void *> mem(io, io + n_io);
+    for (int i = 0; i < warmup; ++i) h->run(mem, false);
+    void *s = h->stream();
+    void *e0 = nullptr, *e1 = nullptr;
+    auto chk = [](int rc) {
+      if (rc) throw std::runtime_error(lt_hip_last_error());
+    };
+    chk(lt_hip_event_create(&e0));
+    chk(lt_hip_event_create(&e1));
+    chk(lt_hip_stream_synchronize(s));
+    chk(lt_hip_event_record(e0, s));
+    for (int i = 0; i < iters; ++i) h->run(mem, false);
+    chk(lt_hip_event_record(e1, s));
+    chk(lt_hip_event_synchronize(e1));
+    float total = 0;
+    chk(lt_hip_event_elapsed_ms(e0, e1, &total));
+    lt_hip_event_destroy(e0);
+    lt_hip_event_destroy(e1);
+    *ms = total / (iters > 0 ? iters : 1);
+    return 0;
+  });
+}
+
+}  // extern "C"

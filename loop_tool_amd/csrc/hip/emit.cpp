@@ -1,0 +1,133 @@
+// Launch-plan construction for the hip backend.
+//
+// Order of preference for each LoopTree:
+//   1. contraction on MFMA (emit_contract.cpp)  - dense matmul / conv nests
+//   2. row-cooperative reduction (emit_rowreduce.cpp) - softmax-like rows
+//   3. pointwise lowering (emit_pointwise.cpp)  - everything else, exact
+// The "hip:plain" annotation (or knobs.force_plain) selects 3 directly.
+#include <sstream>
+
+#include "emit.h"
+#include "emit_internal.h"
+
+namespace loop_tool {
+namespace hip {
+
+const char *kernel_prelude() {
+  return R"(// generated by the loop_tool hip backend for gfx950
+// (hipRTC provides the HIP device runtime implicitly)
+typedef float lt_f4 __attribute__((ext_vector_type(4)));
+typedef float lt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float lt_selmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float lt_selmin(float a, float b) { return a < b ? a : b; }
+)";
+}
+
+namespace {
+
+// A reduction whose value is reused across extra loop vars (or by several
+// consumers) is computed once into scratch by its own stage instead of being
+// recomputed inside every consumer element.
+std::map<IR::NodeRef, int> choose_materialized(const Dataflow &df) {
+  std::map<IR::NodeRef, int> out;
+  int next = 0;
+  for (auto nr : df.topo) {
+    const auto &n = df.nodes.at(nr);
+    if (n.op == Operation::write || n.red_vars.empty()) continue;
+    bool reuse = n.consumers > 1;
+    for (const auto &p : df.nodes) {
+      const auto &c = p.second;
+      for (const auto &in : c.inputs) {
+        if (in.src != nr) continue;
+        if (c.vars.size() + c.red_vars.size() > n.vars.size()) reuse = true;
+      }
+    }
+    if (reuse) out[nr] = next++;
+  }
+  return out;
+}
+
+}  // namespace
+
+Plan build_plan(const Dataflow &df, const Knobs &knobs) {
+  Plan plan;
+  int n_in = df.num_inputs;
+  for (int i = 0; i < n_in; ++i) {
+    auto nr = df.ir().inputs()[i];
+    plan.buffers.push_back({BufferSpec::INPUT, i, df.numel(df.nodes.at(nr).vars), nr});
+  }
+  for (size_t j = 0; j < df.writes.size(); ++j) {
+    auto nr = df.writes[j];
+    plan.buffers.push_back(
+        {BufferSpec::OUTPUT, (int)j, df.numel(df.nodes.at(nr).vars), nr});
+  }
+
+  if (!knobs.force_plain) {
+    if (try_contract(df, knobs, plan)) return plan;
+    if (try_rowreduce(df, knobs, plan)) return plan;
+  }
+
+  auto mat = choose_materialized(df);
+  std::map<int, std::string> input_args;
+  for (int i = 0; i < n_in; ++i) input_args[i] = "in" + std::to_string(i);
+  std::map<IR::NodeRef, int> scratch_buf;
+  for (const auto &p : mat) {
+    scratch_buf[p.first] = (int)plan.buffers.size();
+    plan.buffers.push_back({BufferSpec::SCRATCH, p.second,
+                            df.numel(df.nodes.at(p.first).vars), p.first});
+  }
+
+  std::stringstream src;
+  src << kernel_prelude();
+  std::stringstream desc;
+  desc << "[hip] pointwise lowering, " << mat.size() << " materialised reductions\n";
+  int block = knobs.block > 0 ? knobs.block : 256;
+
+  std::vector<IR::NodeRef> targets;
+  for (auto nr : df.topo)
+    if (mat.count(nr) || df.nodes.at(nr).op == Operation::write) targets.push_back(nr);
+
+  int k = 0;
+  for (auto target : targets) {
+    PointwiseEmitter em(df, mat, input_args);
+    Stage st;
+    st.kernel = "lt_pw" + std::to_string(k++);
+    st.kind = "pointwise";
+    // arguments: every input, every scratch buffer, then the target buffer
+    std::stringstream sig;
+    for (int i = 0; i < n_in; ++i) {
+      sig << "const float* __restrict__ in" << i << ", ";
+      st.buffers.push_back(i);
+    }
+    for (const auto &p : mat) {
+      if (p.first == target) continue;
+      sig << "const float* __restrict__ s" << p.second << ", ";
+      st.buffers.push_back(scratch_buf.at(p.first));
+    }
+    int out_buf;
+    const auto &tn = df.nodes.at(target);
+    if (tn.op == Operation::write)
+      out_buf = n_in + tn.output_index;
+    else
+      out_buf = scratch_buf.at(target);
+    sig << "float* __restrict__ out";
+    st.buffers.push_back(out_buf);
+    st.writes.push_back(out_buf);
+    int64_t threads = 0;
+    src << em.kernel(st.kernel, target, sig.str(), "out", block, &threads) << "\n";
+    int64_t blocks = (threads + block - 1) / block;
+    ASSERT(blocks < (1LL << 31)) << "grid too large";
+    st.grid[0] = (unsigned)blocks;
+    st.block[0] = block;
+    st.bytes = threads * 4;
+    plan.stages.push_back(st);
+    desc << "  stage " << st.kernel << ": " << df.ir().dump(target) << " over "
+         << threads << " elements\n";
+  }
+  plan.source = src.str();
+  plan.description = desc.str();
+  return plan;
+}
+
+}  // namespace hip
+}  // namespace loop_tool

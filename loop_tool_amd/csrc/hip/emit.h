@@ -1,0 +1,58 @@
+// HIP source emitters for the nest classes of the hip backend.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "nest.h"
+
+namespace loop_tool {
+namespace hip {
+
+// One device buffer of a plan: an IR input, an IR output, or backend scratch.
+struct BufferSpec {
+  enum Kind { INPUT, OUTPUT, SCRATCH } kind;
+  int index;       // position among inputs/outputs, or scratch number
+  int64_t numel;   // float elements
+  IR::NodeRef node;
+};
+
+// One kernel launch of a plan.
+struct Stage {
+  std::string kernel;          // extern "C" function name in the module
+  std::string kind;            // "pointwise", "rowreduce", "contract", ...
+  std::vector<int> buffers;    // plan buffer ids, in kernel argument order
+  std::vector<int> writes;     // plan buffer ids the stage writes
+  unsigned grid[3] = {1, 1, 1};
+  unsigned block[3] = {256, 1, 1};
+  unsigned shared_bytes = 0;
+  int64_t bytes = 0;           // algorithmic HBM bytes (read + write)
+  int64_t flops = 0;           // algorithmic flops
+};
+
+struct Plan {
+  std::vector<BufferSpec> buffers;
+  std::vector<Stage> stages;
+  std::string source;          // all kernels of the plan, one module
+  std::string description;     // human-readable summary (Compiled::dump)
+};
+
+// Knobs read from LoopTree annotations (see hip_backend.cpp).
+struct Knobs {
+  bool force_plain = false;    // "hip:plain" - only the pointwise lowering
+  bool bf16 = false;           // "mfma_bf16" operand precision for contractions
+  bool f16 = false;            // "mfma_f16"
+  int block = 256;             // "block=N"
+  int vec = 0;                 // "vec=N" elements per thread (0 = auto)
+  int tile_m = 0, tile_n = 0;  // "tile=MxN" for contractions (0 = auto)
+};
+
+// Builds the whole launch plan for a LoopTree.
+Plan build_plan(const Dataflow &df, const Knobs &knobs);
+
+// Shared helpers for emitters.
+std::string float_literal(double v);
+std::string unary_expr(Operation op, const std::string &x);
+
+}  // namespace hip
+}  // namespace loop_tool

@@ -1,0 +1,374 @@
+// The "hip" backend: loop_tool::Backend / Compiled for AMD Instinct MI355X.
+//
+// Registered beside the reference's CPU backends through the same plugin
+// interface (include/loop_tool/backend.h:52-92; registration pattern of
+// src/backends/cpu/cpp.cpp:505-514).  compile_impl() lowers the LoopTree to a
+// launch plan (emit.cpp), generates one HIP module for it and compiles it for
+// gfx950 with hipRTC through the C ABI of include/lt_hip.h; run() launches the
+// plan's kernels on the Compiled's stream.
+//
+// Buffers handed to run() may be device (HBM) pointers, used in place, or
+// host pointers, staged through device buffers owned by the Compiled (copied
+// in before and out after the launch); either way the call is synchronous
+// when staging is involved.
+//
+// Knobs are read from LoopTree annotations (loop or node), which the
+// reference C backend ignores, so an annotated tree still runs on the oracle:
+//   hip:plain      only the exact pointwise lowering
+//   mfma_bf16      bf16 operands, fp32 accumulation for contractions
+//   mfma_f16       fp16 operands, fp32 accumulation for contractions
+//   block=N        threads per workgroup of pointwise stages
+//   vec=N          elements per lane for row reductions
+//   tile=MxN       workgroup tile of contractions
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <sstream>
+#include <unordered_map>
+
+#include "emit.h"
+#include "loop_tool/backend.h"
+#include "loop_tool/hardware.h"
+#include "lt_hip.h"
+#include "nest.h"
+#include "hip_backend.h"
+
+namespace loop_tool {
+namespace hip {
+
+namespace {
+
+void hip_check(int rc, const char *what) {
+  ASSERT(rc == 0) << what << ": " << lt_hip_last_error();
+}
+
+std::mutex &runtime_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+bool &runtime_ready() {
+  static bool ready = false;
+  return ready;
+}
+
+// Set by an exit handler registered right after the HIP runtime initialises,
+// so it runs before the runtime's own teardown: device resources still held
+// by static caches at process exit are then left to the driver instead of
+// being released into a runtime that is already gone.
+std::atomic<bool> g_exiting{false};
+void mark_exiting() { g_exiting = true; }
+
+uint64_t fnv1a(const std::string &s) {
+  uint64_t h = 1469598103934665603ULL;
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+// In-process and on-disk cache of code objects, keyed by module source.
+struct CodeCache {
+  std::mutex mu;
+  std::unordered_map<std::string, std::shared_ptr<std::vector<char>>> mem;
+  std::string dir;
+  CodeCache() {
+    if (const char *d = std::getenv("LT_HIP_CACHE_DIR")) {
+      dir = d;
+    } else if (const char *h = std::getenv("HOME")) {
+      dir = std::string(h) + "/.cache/loop_tool_amd";
+    }
+    if (!dir.empty()) {
+      std::string cmd = "mkdir -p '" + dir + "' 2>/dev/null";
+      if (std::system(cmd.c_str()) != 0) dir.clear();
+    }
+  }
+};
+
+CodeCache &code_cache() {
+  static CodeCache c;
+  return c;
+}
+
+}  // namespace
+
+void ensure_runtime() {
+  std::lock_guard<std::mutex> g(runtime_mutex());
+  if (runtime_ready()) return;
+  hip_check(lt_hip_init(), "HIP runtime initialisation (is a GPU visible?)");
+  std::atexit(mark_exiting);
+  runtime_ready() = true;
+}
+
+std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
+                                                  const std::string &name,
+                                                  bool *from_cache) {
+  auto &cc = code_cache();
+  {
+    std::lock_guard<std::mutex> g(cc.mu);
+    auto it = cc.mem.find(source);
+    if (it != cc.mem.end()) {
+      if (from_cache) *from_cache = true;
+      return it->second;
+    }
+  }
+  std::string path;
+  if (!cc.dir.empty()) {
+    std::stringstream p;
+    p << cc.dir << "/" << std::hex << fnv1a(source) << "_" << source.size() << ".co";
+    path = p.str();
+    std::ifstream f(path, std::ios::binary);
+    if (f) {
+      auto buf = std::make_shared<std::vector<char>>(
+          (std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+      if (!buf->empty()) {
+        std::lock_guard<std::mutex> g(cc.mu);
+        cc.mem[source] = buf;
+        if (from_cache) *from_cache = true;
+        return buf;
+      }
+    }
+  }
+  void *code = nullptr;
+  size_t size = 0;
+  hip_check(lt_hip_compile(source.c_str(), name.c_str(), nullptr, 0, &code, &size),
+            "hipRTC");
+  auto buf = std::make_shared<std::vector<char>>((char *)code, (char *)code + size);
+  lt_hip_free_code(code);
+  if (!path.empty()) {
+    std::string tmp = path + ".tmp" + std::to_string((uint64_t)(uintptr_t)buf.get());
+    {
+      std::ofstream f(tmp, std::ios::binary);
+      f.write(buf->data(), (std::streamsize)buf->size());
+    }
+    std::rename(tmp.c_str(), path.c_str());
+  }
+  std::lock_guard<std::mutex> g(cc.mu);
+  cc.mem[source] = buf;
+  if (from_cache) *from_cache = false;
+  return buf;
+}
+
+Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
+  Knobs k;
+  k.bf16 = bf16_variant;
+  auto scan = [&](const std::string &annot) {
+    std::stringstream ss(annot);
+    std::string tok;
+    while (ss >> tok) {
+      if (tok == "hip:plain") k.force_plain = true;
+      else if (tok == "mfma_bf16") k.bf16 = true;
+      else if (tok == "mfma_f16") k.f16 = true;
+      else if (tok.rfind("block=", 0) == 0) k.block = std::stoi(tok.substr(6));
+      else if (tok.rfind("vec=", 0) == 0) k.vec = std::stoi(tok.substr(4));
+      else if (tok.rfind("tile=", 0) == 0) {
+        auto x = tok.find('x', 5);
+        if (x != std::string::npos) {
+          k.tile_m = std::stoi(tok.substr(5, x - 5));
+          k.tile_n = std::stoi(tok.substr(x + 1));
+        }
+      }
+    }
+  };
+  for (const auto &a : lt.annotations) scan(a);
+  for (auto n : lt.ir.nodes()) {
+    scan(lt.ir.annotation(n));
+    for (const auto &a : lt.ir.loop_annotations(n)) scan(a);
+  }
+  ASSERT(k.block >= 64 && k.block <= 1024 && k.block % 64 == 0)
+      << "block=N must be a multiple of 64 in [64, 1024]";
+  return k;
+}
+
+// ------------------------------------------------------------ HipCompiled ----
+HipCompiled::HipCompiled(const LoopTree &lt, bool bf16_variant) {
+  auto df = analyze(lt);
+  knobs_ = knobs_from(lt, bf16_variant);
+  plan_ = build_plan(*df, knobs_);
+  bool cached = false;
+  code_ = compile_source(plan_.source, "lt_module.hip", &cached);
+  int_properties["stages"] = (int)plan_.stages.size();
+  int_properties["code_cached"] = cached ? 1 : 0;
+  std::string kinds;
+  for (const auto &s : plan_.stages) kinds += (kinds.empty() ? "" : ",") + s.kind;
+  string_properties["kinds"] = kinds;
+  string_properties["code"] = plan_.source;
+  int64_t bytes = 0, flops = 0;
+  for (const auto &s : plan_.stages) {
+    bytes += s.bytes;
+    flops += s.flops;
+  }
+  algorithmic_bytes_ = bytes;
+  algorithmic_flops_ = flops;
+}
+
+bool runtime_exiting() { return g_exiting.load(); }
+
+HipCompiled::~HipCompiled() {
+  if (!loaded_ || runtime_exiting()) return;
+  for (auto p : scratch_) lt_hip_free(p);
+  for (auto &p : staging_) lt_hip_free(p.second.first);
+  if (stream_) lt_hip_stream_destroy(stream_);
+  if (module_) lt_hip_module_unload(module_);
+}
+
+void HipCompiled::load() const {
+  if (loaded_) return;
+  ensure_runtime();
+  hip_check(lt_hip_module_load(code_->data(), &module_), "module load");
+  for (const auto &s : plan_.stages) {
+    void *fn = nullptr;
+    hip_check(lt_hip_module_get_function(module_, s.kernel.c_str(), &fn), "get function");
+    functions_.push_back(fn);
+  }
+  for (const auto &b : plan_.buffers) {
+    if (b.kind != BufferSpec::SCRATCH) continue;
+    void *p = nullptr;
+    hip_check(lt_hip_malloc(&p, b.numel * sizeof(float)), "scratch allocation");
+    scratch_.push_back(p);
+  }
+  hip_check(lt_hip_stream_create(&stream_), "stream");
+  loaded_ = true;
+}
+
+void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream) const {
+  for (size_t k = 0; k < plan_.stages.size(); ++k) {
+    const auto &s = plan_.stages[k];
+    std::vector<void *> ptrs;
+    for (auto b : s.buffers) ptrs.push_back(device_buffers.at(b));
+    std::vector<void *> args;
+    for (auto &p : ptrs) args.push_back(&p);
+    hip_check(lt_hip_launch(functions_[k], s.grid[0], s.grid[1], s.grid[2], s.block[0],
+                            s.block[1], s.block[2], s.shared_bytes, stream, args.data()),
+              "kernel launch");
+  }
+}
+
+std::vector<void *> HipCompiled::device_buffers(const std::vector<void *> &io) const {
+  std::vector<void *> bufs;
+  size_t scratch_i = 0;
+  for (const auto &b : plan_.buffers) {
+    if (b.kind == BufferSpec::SCRATCH) {
+      bufs.push_back(scratch_[scratch_i++]);
+    } else {
+      size_t pos = b.kind == BufferSpec::INPUT ? b.index : num_inputs() + b.index;
+      bufs.push_back(io.at(pos));
+    }
+  }
+  return bufs;
+}
+
+size_t HipCompiled::num_inputs() const {
+  size_t n = 0;
+  for (const auto &b : plan_.buffers) n += b.kind == BufferSpec::INPUT;
+  return n;
+}
+
+size_t HipCompiled::num_io() const {
+  size_t n = 0;
+  for (const auto &b : plan_.buffers) n += b.kind != BufferSpec::SCRATCH;
+  return n;
+}
+
+void HipCompiled::run(const std::vector<void *> &memory, bool sync) const {
+  std::lock_guard<std::mutex> g(mu_);
+  load();
+  ASSERT(memory.size() >= num_io())
+      << "hip backend expects " << num_io() << " buffers (inputs then outputs), got "
+      << memory.size();
+  std::vector<void *> io(memory.begin(), memory.begin() + num_io());
+  std::vector<std::pair<size_t, void *>> copy_out;  // (io position, host ptr)
+  bool staged = false;
+  for (size_t i = 0; i < io.size(); ++i) {
+    if (!io[i] || lt_hip_is_device_pointer(io[i])) continue;
+    const auto &spec = plan_.buffers[i];
+    size_t bytes = spec.numel * sizeof(float);
+    auto &st = staging_[i];
+    if (st.second < bytes) {
+      if (st.first) lt_hip_free(st.first);
+      hip_check(lt_hip_malloc(&st.first, bytes), "staging allocation");
+      st.second = bytes;
+    }
+    if (spec.kind == BufferSpec::INPUT)
+      hip_check(lt_hip_memcpy_async(st.first, io[i], bytes, 0, stream_), "copy in");
+    else
+      copy_out.emplace_back(i, io[i]);
+    io[i] = st.first;
+    staged = true;
+  }
+  launch(device_buffers(io), stream_);
+  for (const auto &p : copy_out)
+    hip_check(lt_hip_memcpy_async(p.second, io[p.first],
+                                  plan_.buffers[p.first].numel * sizeof(float), 1, stream_),
+              "copy out");
+  if (sync || staged) hip_check(lt_hip_stream_synchronize(stream_), "synchronize");
+}
+
+void HipCompiled::run_on(const std::vector<void *> &device_io,
+                         const std::vector<void *> &scratch, void *stream) const {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    load();
+  }
+  std::vector<void *> bufs;
+  size_t si = 0;
+  for (const auto &b : plan_.buffers) {
+    if (b.kind == BufferSpec::SCRATCH) {
+      ASSERT(si < scratch.size()) << "missing scratch buffer";
+      bufs.push_back(scratch[si++]);
+    } else {
+      size_t pos = b.kind == BufferSpec::INPUT ? b.index : num_inputs() + b.index;
+      bufs.push_back(device_io.at(pos));
+    }
+  }
+  launch(bufs, stream);
+}
+
+std::vector<int64_t> HipCompiled::scratch_sizes() const {
+  std::vector<int64_t> out;
+  for (const auto &b : plan_.buffers)
+    if (b.kind == BufferSpec::SCRATCH) out.push_back(b.numel);
+  return out;
+}
+
+std::string HipCompiled::dump() const {
+  return plan_.description + "\n" + plan_.source;
+}
+
+// ------------------------------------------------------------ registration ----
+struct HipBackend : public Backend {
+  bool bf16;
+  HipBackend(std::string name, bool bf16_) : Backend(std::move(name)), bf16(bf16_) {}
+  std::unique_ptr<Compiled> compile_impl(const LoopTree &lt) const override {
+    return std::make_unique<HipCompiled>(lt, bf16);
+  }
+  // accepts CPU-addressable memory (staged) as well as device pointers
+  int hardware_requirement() const override { return 1 << 0; }
+};
+
+// Pinned host memory that kernels can also address; the lazy frontend can
+// hand its pointers straight to users (include/loop_tool/hardware.h:24-33).
+struct HipHardware : public Hardware {
+  HipHardware() : Hardware("hip", 1) {}
+  Memory alloc(size_t size) override {
+    ensure_runtime();
+    void *p = nullptr;
+    hip_check(lt_hip_host_malloc(&p, size), "pinned allocation");
+    return Memory{(1 << 0) | (1 << id()), p};
+  }
+  void free(Memory &data) override {
+    if (data.address && !runtime_exiting()) lt_hip_host_free(data.address);
+    data.address = nullptr;
+    data.compatible = 0;
+  }
+};
+
+static RegisterBackend register_hip(std::make_shared<HipBackend>("hip", false));
+static RegisterBackend register_hip_bf16(std::make_shared<HipBackend>("hip_bf16", true));
+static RegisterHardware register_hip_hw(std::make_shared<HipHardware>());
+
+}  // namespace hip
+}  // namespace loop_tool

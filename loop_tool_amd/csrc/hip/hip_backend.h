@@ -1,0 +1,71 @@
+// HipCompiled: the Compiled object of the "hip" backend (see hip_backend.cpp).
+#pragma once
+
+#include <memory>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "emit.h"
+#include "loop_tool/backend.h"
+#include "lt_hip.h"
+
+namespace loop_tool {
+namespace hip {
+
+void ensure_runtime();
+// true once the process has started exiting (device frees are skipped)
+bool runtime_exiting();
+std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
+                                                  const std::string &name,
+                                                  bool *from_cache);
+Knobs knobs_from(const LoopTree &lt, bool bf16_variant);
+
+class HipCompiled : public Compiled {
+ public:
+  HipCompiled(const LoopTree &lt, bool bf16_variant);
+  ~HipCompiled() override;
+
+  void run(const std::vector<void *> &memory, bool sync = true) const override;
+  std::string dump() const override;
+
+  // Launch on a caller stream with caller-owned device buffers (inputs then
+  // outputs) and scratch (scratch_sizes() floats each); nothing is
+  // allocated, copied or synchronised.  Safe to call concurrently when every
+  // caller brings its own scratch.
+  void run_on(const std::vector<void *> &device_io, const std::vector<void *> &scratch,
+              void *stream) const;
+  std::vector<int64_t> scratch_sizes() const;
+
+  const Plan &plan() const { return plan_; }
+  int64_t algorithmic_bytes() const { return algorithmic_bytes_; }
+  int64_t algorithmic_flops() const { return algorithmic_flops_; }
+  void *stream() const {
+    load();
+    return stream_;
+  }
+  size_t num_inputs() const;
+  size_t num_io() const;
+
+ private:
+  void load() const;
+  void launch(const std::vector<void *> &device_buffers, void *stream) const;
+  std::vector<void *> device_buffers(const std::vector<void *> &io) const;
+
+  Plan plan_;
+  Knobs knobs_;
+  std::shared_ptr<std::vector<char>> code_;
+  int64_t algorithmic_bytes_ = 0;
+  int64_t algorithmic_flops_ = 0;
+
+  mutable std::mutex mu_;
+  mutable bool loaded_ = false;
+  mutable lt_hip_module module_ = nullptr;
+  mutable std::vector<void *> functions_;
+  mutable std::vector<void *> scratch_;
+  mutable void *stream_ = nullptr;
+  mutable std::unordered_map<size_t, std::pair<void *, size_t>> staging_;
+};
+
+}  // namespace hip
+}  // namespace loop_tool

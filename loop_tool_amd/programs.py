@@ -1,0 +1,149 @@
+"""The benchmark programs, written with the lazy frontend, plus their seeded
+synthetic inputs.
+
+Every builder takes a `capi.Library` (so the same program can be built with
+any implementation of the C ABI) and returns (output tensor, input shapes);
+the IR inputs are the leaf tensors in creation order.
+
+Inputs follow a counter-based generator so every implementation sees the same
+bits: SplitMix64 over (seed, element index); uniform [-1, 1) from the top 24
+bits; normal by Box-Muller in float64, rounded to float32.
+"""
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, n):
+    """n outputs of SplitMix64 seeded with `seed` (state += golden per draw)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + i * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed, shape):
+    n = int(np.prod(shape))
+    x = splitmix64(seed, n)
+    u = (x >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+    return (u * 2.0 - 1.0).astype(np.float32).reshape(shape)
+
+
+def normal(seed, shape, std=1.0):
+    n = int(np.prod(shape))
+    m = (n + 1) // 2
+    x = splitmix64(seed, 2 * m)
+    u1 = ((x[0::2] >> np.uint64(11)).astype(np.float64) + 1.0) * (2.0 ** -53)
+    u2 = (x[1::2] >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m, np.float64)
+    z[0::2] = r * np.cos(2 * np.pi * u2)
+    z[1::2] = r * np.sin(2 * np.pi * u2)
+    return (z[:n] * std).astype(np.float32).reshape(shape)
+
+
+# ---------------------------------------------------------------- programs --
+def matmul(L, M, N, K):
+    """C[m,n] = sum_k A[m,k] * B[k,n]  (config 1 of the benchmark suite)."""
+    m, n, k = L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
+    A = L.Tensor(M, K)
+    B = L.Tensor(K, N)
+    C = (A.as_(m, k) * B.as_(k, n)).sum(k)
+    return C, [(M, K), (K, N)]
+
+
+def softmax(L, R, Cn):
+    """Row softmax over C: max-reduce, subtract, exp, sum-reduce, divide."""
+    r, c = L.Symbol("R"), L.Symbol("C")
+    X = L.Tensor(R, Cn).as_(r, c)
+    mx = X.max_reduce(c)
+    e = (X - mx).exp()
+    Y = e / e.sum(c)
+    return Y, [(R, Cn)]
+
+
+def conv2d(L, N, Ci, Co, H, W, k=3, pad=1, stride=1):
+    """Y[n,co,ho,wo] = sum_{ci,kh,kw} Xpad[n,ci,ho*s+kh,wo*s+kw] * W[co,ci,kh,kw]
+    with Xpad the zero-padded view of X (a pad nest plus a window sum)."""
+    n, ci, h, w = L.Symbol("N"), L.Symbol("Ci"), L.Symbol("H"), L.Symbol("W")
+    co, kh, kw = L.Symbol("Co"), L.Symbol("Kh"), L.Symbol("Kw")
+    X = L.Tensor(N, Ci, H, W).as_(n, ci, h, w)
+    Wt = L.Tensor(Co, Ci, k, k).as_(co, ci, kh, kw)
+    if pad:
+        X = X.pad(h, pad).pad(w, pad)
+        sh, sw = X.shape[2], X.shape[3]
+    else:
+        sh, sw = h, w
+    Y = L.convolve(X, Wt, [sh, sw], [kh, kw], stride)
+    # convolve yields [N, Ho, Wo, Co]; lay the output out as NCHW
+    s = Y.shape
+    Y = Y.transpose(s[0], s[3], s[1], s[2])
+    return Y, [(N, Ci, H, W), (Co, Ci, k, k)]
+
+
+def maxpool2d(L, N, C, H, W, k=2, stride=2):
+    n, c, h, w = L.Symbol("N"), L.Symbol("C"), L.Symbol("H"), L.Symbol("W")
+    X = L.Tensor(N, C, H, W).as_(n, c, h, w)
+    return L.maxpool(X, [h, w], k, stride), [(N, C, H, W)]
+
+
+def pad1d(L, N, pre, post):
+    n = L.Symbol("N")
+    X = L.Tensor(N).as_(n)
+    return X.pad(n, pre, post), [(N,)]
+
+
+def concat2d(L, B, M0, M1):
+    b, m0, m1 = L.Symbol("N"), L.Symbol("M0"), L.Symbol("M1")
+    A = L.Tensor(B, M0).as_(b, m0)
+    Bt = L.Tensor(B, M1).as_(b, m1)
+    return A | Bt, [(B, M0), (B, M1)]
+
+
+def transpose2d(L, M, N):
+    m, n = L.Symbol("M"), L.Symbol("N")
+    A = L.Tensor(M, N).as_(m, n)
+    return A.to([n, m]), [(M, N)]
+
+
+def add_broadcast(L, M, N):
+    m, n = L.Symbol("M"), L.Symbol("N")
+    A = L.Tensor(M, N).as_(m, n)
+    b = L.Tensor(N).as_(n)
+    return A + b, [(M, N), (N,)]
+
+
+def reference_output(name, inputs, **kw):
+    """float64 evaluation of a program (closed forms, not the oracle)."""
+    x = [np.asarray(a, np.float64) for a in inputs]
+    if name == "matmul":
+        return x[0] @ x[1]
+    if name == "softmax":
+        e = np.exp(x[0] - x[0].max(axis=1, keepdims=True))
+        return e / e.sum(axis=1, keepdims=True)
+    if name == "conv2d":
+        X, W = x
+        pad, stride = kw.get("pad", 1), kw.get("stride", 1)
+        Xp = np.pad(X, ((0, 0), (0, 0), (pad, pad), (pad, pad)))
+        k = W.shape[2]
+        Ho = (Xp.shape[2] - k) // stride + 1
+        Wo = (Xp.shape[3] - k) // stride + 1
+        Y = np.zeros((X.shape[0], W.shape[0], Ho, Wo))
+        for a in range(k):
+            for b in range(k):
+                patch = Xp[:, :, a:a + stride * Ho:stride, b:b + stride * Wo:stride]
+                Y += np.einsum("nchw,oc->nohw", patch, W[:, :, a, b])
+        return Y
+    if name == "maxpool2d":
+        X = x[0]
+        k, s = kw.get("k", 2), kw.get("stride", 2)
+        Ho = (X.shape[2] - k) // s + 1
+        Wo = (X.shape[3] - k) // s + 1
+        out = np.full(X.shape[:2] + (Ho, Wo), -np.inf)
+        for a in range(k):
+            for b in range(k):
+                out = np.maximum(out, X[:, :, a:a + s * Ho:s, b:b + s * Wo:s])
+        return out
+    raise KeyError(name)

@@ -1,0 +1,44 @@
+"""pytest configuration: the `gpu` marker and shared fixtures.
+
+The oracle libraries (oracle/) are test infrastructure: only tests, smoke()
+and the benchmark's CPU-baseline leg load them.
+"""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liblt_oracle.so")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libloop_tool_ref.so")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (MI355X)")
+
+
+@pytest.fixture(scope="session")
+def lt():
+    import loop_tool_amd as lt_mod
+    if "cpp" not in lt_mod.backends():
+        lt_mod.load_lib(ORACLE_LIB)
+    return lt_mod.lib
+
+
+@pytest.fixture(scope="session")
+def ref():
+    if not os.path.exists(REF_LIB):
+        pytest.skip("reference library oracle/_ref not built")
+    from loop_tool_amd import capi
+    return capi.Library(REF_LIB)
+
+
+@pytest.fixture(scope="session")
+def hip_dev():
+    import loop_tool_amd.hip as hip
+    if not hip.available():
+        pytest.skip("no HIP device")
+    hip.init()
+    return hip

@@ -1,0 +1,74 @@
+"""GPU parity of the hip backend against the C oracle ("cpp", a restatement of
+the reference C backend) on the same seeded inputs, through the C ABI.
+
+Bars: bit-exact for the plain (pointwise) lowering of + - * / max min and
+index-only nests; <= 1e-5 normwise relative for FP32 arithmetic nests lowered
+with a different accumulation order; every result is also compared with a
+float64 evaluation of the same program and its error is printed.
+"""
+import numpy as np
+import pytest
+
+from loop_tool_amd import programs as P
+
+pytestmark = pytest.mark.gpu
+
+
+def run_backend(L, out, shapes, inputs, backend, annotate=None):
+    lt = out.loop_tree
+    if annotate:
+        lt = lt.annotate(lt.roots[0], annotate)
+    cc = L.compile(lt, backend)
+    n_out = lt.io_numel()[len(inputs)]
+    y = np.zeros(n_out, np.float32)
+    cc.run([np.ascontiguousarray(a, np.float32) for a in inputs] + [y])
+    return y, cc
+
+
+def normwise(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+CASES = [
+    ("matmul", dict(M=128, N=128, K=128), lambda s: [P.uniform(1, s[0]), P.uniform(2, s[1])]),
+    ("matmul", dict(M=37, N=53, K=29), lambda s: [P.uniform(1, s[0]), P.uniform(2, s[1])]),
+    ("softmax", dict(R=64, Cn=1000), lambda s: [P.normal(3, s[0], 3.0)]),
+    ("conv2d", dict(N=2, Ci=8, Co=16, H=12, W=12), lambda s: [P.normal(4, s[0]), P.normal(5, s[1])]),
+    ("maxpool2d", dict(N=2, C=3, H=16, W=16), lambda s: [P.normal(6, s[0])]),
+]
+
+
+@pytest.mark.parametrize("name,kw,gen", CASES, ids=[c[0] + str(i) for i, c in enumerate(CASES)])
+def test_plain_hip_bit_exact_vs_oracle(lt, hip_dev, name, kw, gen):
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = gen(shapes)
+    y_ref, _ = run_backend(lt, out, shapes, inputs, "cpp")
+    y_hip, cc = run_backend(lt, out, shapes, inputs, "hip", annotate="hip:plain")
+    f64 = P.reference_output(name, inputs).ravel()
+    print(name, kw, "oracle err vs f64 %.3g, hip err vs f64 %.3g" % (
+        normwise(y_ref, f64), normwise(y_hip, f64)))
+    if name == "softmax":
+        # exp is double libm on the host and ocml on the device: equal after
+        # rounding to float except in double-rounding corner cases
+        assert normwise(y_hip, y_ref) <= 1e-6
+    else:
+        assert np.array_equal(y_hip.view(np.uint32), y_ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("R,Cn", [(512, 8192), (100, 1000), (33, 130), (8, 65536)])
+def test_rowreduce_softmax_vs_oracle(lt, hip_dev, R, Cn):
+    """Default hip lowering of softmax (row-cooperative kernel): tree-ordered
+    sums, so the bar is 1e-5 normwise relative to the C oracle."""
+    out, shapes = P.softmax(lt, R, Cn)
+    x = P.normal(3, shapes[0], 3.0)
+    y_ref, _ = run_backend(lt, out, shapes, [x], "cpp")
+    y_hip, cc = run_backend(lt, out, shapes, [x], "hip")
+    f64 = P.reference_output("softmax", [x]).ravel()
+    e_or, e_hip = normwise(y_ref, f64), normwise(y_hip, f64)
+    print("softmax %dx%d kinds=%s oracle-vs-f64 %.3g hip-vs-f64 %.3g hip-vs-oracle %.3g" % (
+        R, Cn, cc.string_property("kinds"), e_or, e_hip, normwise(y_hip, y_ref)))
+    assert normwise(y_hip, y_ref) <= 1e-5
+    rows = y_hip.reshape(R, Cn).astype(np.float64).sum(axis=1)
+    assert np.abs(rows - 1).max() < 1e-4  # rows of a softmax sum to 1
