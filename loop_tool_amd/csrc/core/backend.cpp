@@ -20,9 +20,12 @@ std::unordered_map<std::string, std::shared_ptr<Backend>> &mutable_backends() {
   static std::unordered_map<std::string, std::shared_ptr<Backend>> backends;
   return backends;
 }
+// Plugin libraries stay mapped until the process ends: the registry holds
+// backends whose code lives in them, and static destruction order between
+// this list and the registry is not ours to choose.
 std::list<DynamicLibrary> &loaded_libs() {
-  static std::list<DynamicLibrary> libs;
-  return libs;
+  static auto *libs = new std::list<DynamicLibrary>();
+  return *libs;
 }
 }  // namespace
 

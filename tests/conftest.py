@@ -24,6 +24,8 @@ def lt():
     import loop_tool_amd as lt_mod
     if "cpp" not in lt_mod.backends():
         lt_mod.load_lib(ORACLE_LIB)
+    # lazy `.numpy()` in tests resolves on the oracle; hip runs are explicit
+    lt_mod.set_default_backend("cpp")
     return lt_mod.lib
 
 

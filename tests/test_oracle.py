@@ -1,0 +1,227 @@
+"""Pins the CPU oracle (oracle/cpp_backend.cpp over this repository's loop_tool
+core) against
+  * the reference library compiled from /root/reference (oracle/_ref):
+    identical LoopTrees and bit-identical outputs on the same seeded inputs;
+  * the reference's own known-answer tests (test/test_lazy.cpp,
+    test/test_cpp.cpp, test/test_ir.cpp), re-expressed through the C ABI;
+  * closed forms in float64 (matmul, softmax, conv, max-pool).
+"""
+import re
+
+import numpy as np
+import pytest
+
+from loop_tool_amd import programs as P
+
+
+def norm_ids(s):
+    return re.sub(r"_\d+", "", s)
+
+
+def run(L, out, inputs, backend="cpp"):
+    lt = out.loop_tree
+    cc = L.compile(lt, backend)
+    y = np.zeros(lt.io_numel()[len(inputs)], np.float32)
+    cc.run([np.ascontiguousarray(a, np.float32) for a in inputs] + [y])
+    return y, lt
+
+
+PROGRAMS = [
+    ("matmul", dict(M=16, N=16, K=16), lambda s: [P.uniform(1, s[0]), P.uniform(2, s[1])]),
+    ("matmul", dict(M=37, N=53, K=29), lambda s: [P.uniform(1, s[0]), P.uniform(2, s[1])]),
+    ("softmax", dict(R=16, Cn=300), lambda s: [P.normal(3, s[0], 3.0)]),
+    ("conv2d", dict(N=2, Ci=4, Co=5, H=9, W=7), lambda s: [P.normal(4, s[0]), P.normal(5, s[1])]),
+    ("conv2d", dict(N=1, Ci=3, Co=2, H=9, W=9, pad=0, stride=2),
+     lambda s: [P.normal(4, s[0]), P.normal(5, s[1])]),
+    ("maxpool2d", dict(N=2, C=3, H=8, W=10), lambda s: [P.normal(6, s[0])]),
+    ("pad1d", dict(N=11, pre=2, post=3), lambda s: [P.normal(7, s[0])]),
+    ("concat2d", dict(B=3, M0=5, M1=4), lambda s: [P.normal(8, s[0]), P.normal(9, s[1])]),
+    ("transpose2d", dict(M=7, N=5), lambda s: [P.normal(10, s[0])]),
+    ("add_broadcast", dict(M=6, N=9), lambda s: [P.normal(11, s[0]), P.normal(12, s[1])]),
+]
+
+
+@pytest.mark.parametrize("name,kw,gen", PROGRAMS, ids=["%s%d" % (p[0], i) for i, p in enumerate(PROGRAMS)])
+def test_oracle_matches_reference_library(lt, ref, name, kw, gen):
+    out_a, shapes = getattr(P, name)(lt, **kw)
+    out_b, _ = getattr(P, name)(ref, **kw)
+    inputs = gen(shapes)
+    ya, lta = run(lt, out_a, inputs)
+    yb, ltb = run(ref, out_b, inputs)
+    assert norm_ids(lta.dump()) == norm_ids(ltb.dump())
+    assert lta.flops() == ltb.flops()
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32))
+
+
+@pytest.mark.parametrize("name,kw,gen", PROGRAMS[:6], ids=["%s%d" % (p[0], i) for i, p in enumerate(PROGRAMS[:6])])
+def test_oracle_closed_forms(lt, name, kw, gen):
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = gen(shapes)
+    y, _ = run(lt, out, inputs)
+    extra = {k: v for k, v in kw.items() if k in ("pad", "stride")}
+    f64 = P.reference_output(name, inputs, **extra).ravel()
+    err = np.abs(y - f64).max() / np.abs(f64).max()
+    assert err < 1e-5, err
+    if name == "maxpool2d":  # each output is one of its window's elements
+        assert np.array_equal(y, f64.astype(np.float32))
+
+
+def test_oracle_detects_planted_mistake(lt):
+    """A wrong oracle (here: the matmul output of a transposed operand) fails
+    the float64 closed-form check used above."""
+    out, shapes = P.matmul(lt, 8, 8, 8)
+    a, b = P.uniform(1, shapes[0]), P.uniform(2, shapes[1])
+    y, _ = run(lt, out, [a, b.T.copy()])
+    f64 = P.reference_output("matmul", [a, b]).ravel()
+    assert np.abs(y - f64).max() / np.abs(f64).max() > 1e-3
+
+
+# ---- known answers of the reference's own tests --------------------------------
+def both(lt, ref):
+    return [lt, ref]
+
+
+def test_lazy_conv_known_answer(lt, ref):
+    """test/test_lazy.cpp:302-330 (LazyConv): Y[3] == 3 with all-ones."""
+    for L in both(lt, ref):
+        N, No, K = L.Symbol("N"), L.Symbol("N_o"), L.Symbol("K")
+        A = L.Tensor(N)
+        W = L.Tensor(K)
+        X = A.to([No, K], [(N, No + K)])
+        Y = (X * W).sum(K)
+        Y.set_size(8)
+        W.bind(np.ones(3, np.float32))
+        A.bind(np.ones(10, np.float32))
+        assert A.size(0) == 10
+        assert Y.numpy()[3] == 3
+
+
+def test_lazy_conv_stride_known_answer(lt, ref):
+    """test/test_lazy.cpp:332-354 (LazyConvStride): |A| inferred 17, Y[3]==3."""
+    for L in both(lt, ref):
+        N, No, K = L.Symbol("N"), L.Symbol("N_o"), L.Symbol("K")
+        A = L.Tensor(N)
+        W = L.Tensor(K)
+        X = A.to([No, K], [(N, L.Expr(2) * No + K)])
+        Y = (X * W).sum(K)
+        Y.set_size(8)
+        W.bind(np.ones(3, np.float32))
+        a = np.zeros(17, np.float32)
+        a[:10] = 1
+        A.bind(a)
+        assert A.size(0) == 17
+        assert Y.numpy()[3] == 3
+
+
+def test_lazy_concat_known_answers(lt, ref):
+    """test/test_lazy.cpp:356-414 (LazyConcat1D / LazyConcat2D)."""
+    for L in both(lt, ref):
+        N, M = L.Symbol("N"), L.Symbol("M")
+        A = L.Tensor(N).bind(np.arange(8, dtype=np.float32))
+        B = L.Tensor(M).bind(np.arange(5, dtype=np.float32))
+        C = A | B
+        assert C.size(0) == 13
+        c = C.numpy()
+        assert c[10] == 2
+        assert np.array_equal(c, np.concatenate([np.arange(8), np.arange(5)]).astype(np.float32))
+        N2, M0, M1 = L.Symbol("N"), L.Symbol("M0"), L.Symbol("M1")
+        A2 = L.Tensor(N2, M0)
+        B2 = L.Tensor(N2, M1)
+        C2 = A2 | B2
+        A2.bind(np.full((2, 5), 11, np.float32))
+        B2.bind(np.full((2, 3), 7, np.float32))
+        assert C2.size(1) == 8
+        c2 = C2.numpy().ravel()
+        assert c2[6] == 7 and c2[8] == 11
+
+
+def test_lazy_pad_known_answer(lt, ref):
+    """test/test_lazy.cpp:416-434 (LazyPadOnly): size 7, X_pad[2]==1, X_pad[6]==0."""
+    for L in both(lt, ref):
+        N, Np = L.Symbol("N"), L.Symbol("Np")
+        X = L.Tensor(N)
+        Xp = X.to([Np], [(Np, N + 1), (L.Size(Np), L.Size(N) + 2)])
+        X.bind(np.arange(5, dtype=np.float32))
+        assert Xp.size(0) == 7
+        xp = Xp.numpy()
+        assert xp[2] == 1 and xp[6] == 0 and xp[0] == 0
+        assert np.array_equal(xp[1:6], np.arange(5))
+
+
+def test_lazy_transpose_known_answer(lt, ref):
+    """test/test_lazy.cpp:258-282 (LazyTranspose): flatten-by-constraint of
+    a transposed index gives B[i] == i."""
+    for L in both(lt, ref):
+        M, N, K = L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
+        A = L.Tensor(M, N)
+        B = A.to([K], [(K, N * L.Size(M) + M)])
+        a = np.array([(i % 8) * 8 + i // 8 for i in range(64)], np.float32).reshape(8, 8)
+        A.bind(a)
+        assert B.size(0) == 64
+        assert np.array_equal(B.numpy(), np.arange(64, dtype=np.float32))
+
+
+def test_lazy_stack_and_fill_known_answers(lt, ref):
+    """test/test_lazy.cpp:221-256 (LazyStackMemory D[7]==14, LazyFill C[7]==9)."""
+    for L in both(lt, ref):
+        B = L.Tensor(128).bind(np.arange(128, dtype=np.float32))
+        S = L.Tensor(data=np.array([2.0], np.float32))
+        D = B * S
+        assert D.numpy().ravel()[7] == 14
+        A = L.Tensor(1).bind(np.array([4.5], np.float32))
+        Mm, Nn = L.Symbol("M"), L.Symbol("N")
+        Bb = L.Tensor(Mm, Nn)
+        Bb.bind(np.full((8, 8), 2, np.float32))
+        C = A * Bb
+        assert abs(C.numpy().ravel()[7] - 9) < 1e-3
+
+
+def test_cpp_from_lazy_known_answer(lt, ref):
+    """test/test_cpp.cpp:15-66 (CppFromLazy): A[i]=3i, B[i]=100-2i, 16^3 mm."""
+    a = (np.arange(256) * 3).astype(np.float32).reshape(16, 16)
+    b = (100 - np.arange(256) * 2).astype(np.float32).reshape(16, 16)
+    for L in both(lt, ref):
+        out, _ = P.matmul(L, 16, 16, 16)
+        y, _ = run(L, out, [a, b])
+        assert np.abs(y - (a.astype(np.float64) @ b).ravel()).max() < 0.01
+
+
+def test_cpp_with_tail_known_answer(lt, ref):
+    """test/test_cpp.cpp:68-125 (CppWithTail): split the inner loop by 10
+    (tail 6) and swap it outward; the result is unchanged."""
+    a = (np.arange(256) * 3).astype(np.float32).reshape(16, 16)
+    b = (100 - np.arange(256) * 2).astype(np.float32).reshape(16, 16)
+    dumps = []
+    for L in both(lt, ref):
+        out, _ = P.matmul(L, 16, 16, 16)
+        t = out.loop_tree
+        r = t.children(t.roots[0])[0]
+        t = t.split(r, 10)
+        a_ = t.children(t.children(t.roots[0])[0])[0]
+        b_ = t.children(a_)[0]
+        t = t.swap_loops(a_, b_)
+        dumps.append(norm_ids(t.dump()))
+        cc = L.compile(t, "cpp")
+        y = np.zeros(256, np.float32)
+        cc.run([a, b, y])
+        assert np.abs(y - (a.astype(np.float64) @ b).ravel()).max() < 0.01
+    assert dumps[0] == dumps[1]
+
+
+def test_cpp_view_known_answer(lt, ref):
+    """test/test_cpp.cpp:127-184 (CppView): padded 1-D conv."""
+    x = (np.arange(16) * 3 + 1).astype(np.float32)
+    w = (1 - np.arange(3) * 2).astype(np.float32)
+    expect = np.zeros(16)
+    for i in range(16):
+        for k in range(3):
+            if 0 <= i + k - 1 < 16:
+                expect[i] += x[i + k - 1] * w[k]
+    for L in both(lt, ref):
+        n, npad, no, kk = L.Symbol("n"), L.Symbol("np"), L.Symbol("no"), L.Symbol("k")
+        X = L.Tensor(16)
+        Wt = L.Tensor(3)
+        Xp = X.as_(n).pad(n, 1).as_(npad)
+        Y = (Xp.to([no, kk], [(npad, no + kk)]) * Wt.as_(kk)).sum(kk)
+        y, _ = run(L, Y, [x, w])
+        assert np.abs(y - expect).max() < 0.01
