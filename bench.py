@@ -22,6 +22,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E datasheet
 MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f32": 157.3}
+# HBM bytes per launch of the dominant kernel, from FETCH_SIZE / WRITE_SIZE
+# counter runs scaled by a copy kernel of known size
+# (profiles/lt_row0_softmax_8192x8192_hbm_traffic.md)
+MEASURED_TRAFFIC = {"softmax": 537182208}
 
 
 def parse():
@@ -58,6 +62,13 @@ def workload(lt, name):
                    N=N, C_in=Ci, C_out=Co, H=H, W=H, operands="bf16" if bf else "fp32")
         return out, shapes, gen, cfg, "mfma", "bf16" if bf else "f32", (
             "hip_bf16" if bf else "hip")
+    if name == "copy":
+        R, C = 8192, 8192
+        r, c = lt.Symbol("R"), lt.Symbol("C")
+        out = lt.Tensor(R, C).as_(r, c)
+        gen = lambda: [P.normal(3, (R, C))]
+        return out, [(R, C)], gen, dict(workload="fp32 copy (calibration of HBM counters)",
+                                         rows=R, cols=C), "hbm", "f32", "hip"
     if name == "maxpool":
         out, shapes = P.maxpool2d(lt, 32, 128, 112, 112)
         gen = lambda: [P.normal(6, shapes[0])]
@@ -163,7 +174,8 @@ def main():
         unit = "GB/s"
         achieved = bytes_step / (kern_ms * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=MEASURED_TRAFFIC.get(args.nest))
         metric = "HBM GB/s (algorithmic bytes / time), hip backend"
     else:
         value = flops_step * args.steps / el / 1e12
