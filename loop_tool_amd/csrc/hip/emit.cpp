@@ -82,6 +82,9 @@ Plan build_plan(const Dataflow &df, const Knobs &knobs) {
   std::stringstream desc;
   desc << "[hip] pointwise lowering, " << mat.size() << " materialised reductions\n";
   int block = knobs.block > 0 ? knobs.block : 256;
+  // 32-bit index math when every buffer (and so every in-range index) fits
+  bool int32 = true;
+  for (const auto &b : plan.buffers) int32 &= b.numel < (1LL << 30);
 
   std::vector<IR::NodeRef> targets;
   for (auto nr : df.topo)
@@ -114,12 +117,16 @@ Plan build_plan(const Dataflow &df, const Knobs &knobs) {
     st.buffers.push_back(out_buf);
     st.writes.push_back(out_buf);
     int64_t threads = 0;
-    src << em.kernel(st.kernel, target, sig.str(), "out", block, &threads) << "\n";
+    src << em.kernel(st.kernel, target, sig.str(), "out", block, &threads, knobs.vec, int32)
+        << "\n";
     int64_t blocks = (threads + block - 1) / block;
     ASSERT(blocks < (1LL << 31)) << "grid too large";
     st.grid[0] = (unsigned)blocks;
     st.block[0] = block;
-    st.bytes = threads * 4;
+    // algorithmic bytes: every input element read once, the target written once
+    int64_t bytes = df.numel(df.nodes.at(target).vars) * 4;
+    for (int i = 0; i < n_in; ++i) bytes += df.numel(df.nodes.at(df.ir().inputs()[i]).vars) * 4;
+    st.bytes = bytes;
     plan.stages.push_back(st);
     desc << "  stage " << st.kernel << ": " << df.ir().dump(target) << " over "
          << threads << " elements\n";

@@ -22,7 +22,7 @@ class PointwiseEmitter {
   // compute node) into `out_arg`.  Returns the source; *threads = elements.
   std::string kernel(const std::string &name, IR::NodeRef target,
                      const std::string &signature, const std::string &out_arg,
-                     int block, int64_t *threads);
+                     int block, int64_t *threads, int vec, bool int32);
 
   struct Impl;
 

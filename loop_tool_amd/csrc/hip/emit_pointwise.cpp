@@ -113,6 +113,7 @@ struct PointwiseEmitter::Impl {
   std::map<int, int64_t> extents;                    // symbol id -> extent
   std::map<int, std::string> names;                  // symbol id -> C name
   int counter = 0;
+  std::string ity = "long";                          // index type
 
   Impl(const Dataflow &d, const std::map<IR::NodeRef, int> &m,
        const std::map<int, std::string> &a)
@@ -328,7 +329,7 @@ struct PointwiseEmitter::Impl {
       auto name = fresh("i");
       names[it.id()] = name;
       extents[it.id()] = size;
-      auto *loop = b->child("for (long " + name + " = 0; " + name + " < " +
+      auto *loop = b->child("for (" + ity + " " + name + " = 0; " + name + " < " +
                                 std::to_string(size) + "; ++" + name + ") {",
                             "}");
       loop->bound.insert(it.id());
@@ -365,49 +366,75 @@ PointwiseEmitter::~PointwiseEmitter() = default;
 std::string PointwiseEmitter::kernel(const std::string &name, IR::NodeRef target,
                                      const std::string &signature,
                                      const std::string &out_arg, int block,
-                                     int64_t *threads) {
+                                     int64_t *threads, int vec, bool int32) {
   auto &I = *impl_;
   const auto &df = I.df;
   const auto &n = df.nodes.at(target);
+  I.ity = int32 ? "int" : "long";
+  const std::string ity = I.ity;
   // the target's element space: a write covers its vars, a materialised node
-  // its own vars
+  // its own vars.  Each lane owns E consecutive elements of the contiguous
+  // (last) var and stores them with one vector store.
   std::vector<IR::VarRef> vars = n.vars;
   int64_t total = df.numel(vars);
-  *threads = total;
-
-  Block root;
-  std::vector<Expr> coords;
-  std::stringstream pre;
-  pre << "const long o = (long)blockIdx.x * " << block << " + threadIdx.x;\n";
-  pre << "if (o >= " << total << "L) return;\n";
-  pre << "long t = o;\n";
-  std::vector<std::string> cnames(vars.size());
-  for (int d = (int)vars.size() - 1; d >= 0; --d) {
-    auto s = df.sym(vars[d]);
-    auto cn = "c" + std::to_string(d);
-    cnames[d] = cn;
-    I.names[s.id()] = cn;
-    I.extents[s.id()] = df.size(vars[d]);
-    root.bound.insert(s.id());
-    if (d > 0) {
-      pre << "const long " << cn << " = t % " << df.size(vars[d]) << "L; t /= "
-          << df.size(vars[d]) << "L;\n";
-    } else {
-      pre << "const long " << cn << " = t;\n";
+  int64_t last = vars.empty() ? 1 : df.size(vars.back());
+  int E = 1;
+  for (int cand : {4, 2}) {
+    if (vec > 0 && cand > vec) continue;
+    if (last % cand == 0) {
+      E = cand;
+      break;
     }
   }
-  for (auto v : vars) coords.emplace_back(Expr(df.sym(v)));
+  const int64_t lanes = total / E;
+  *threads = lanes;
 
-  std::string result;
-  if (n.op == Operation::write) {
-    const auto &in = n.inputs.at(0);
-    std::map<int, Expr> subst;
-    for (size_t i = 0; i < vars.size(); ++i) subst[df.sym(vars[i]).id()] = coords[i];
-    result = I.value(&root, in.src, I.input_coords(n, in, subst));
-  } else {
-    result = I.compute(&root, n, coords);
+  Block root;
+  std::stringstream pre;
+  pre << "const " << ity << " o = (" << ity << ")blockIdx.x * " << block
+      << " + (" << ity << ")threadIdx.x;\n";
+  pre << "if (o >= " << lanes << ") return;\n";
+  pre << ity << " t = o;\n";
+  std::vector<Expr> base;  // per var: its coordinate symbol, or the lane's chunk
+  Symbol q("q");
+  for (int d = (int)vars.size() - 1; d >= 0; --d) {
+    bool is_last = d == (int)vars.size() - 1;
+    Symbol s = is_last ? q : df.sym(vars[d]);
+    int64_t extent = is_last ? last / E : df.size(vars[d]);
+    auto cn = "c" + std::to_string(d);
+    I.names[s.id()] = cn;
+    I.extents[s.id()] = extent;
+    root.bound.insert(s.id());
+    if (d > 0)
+      pre << "const " << ity << " " << cn << " = t % " << extent << "; t /= " << extent
+          << ";\n";
+    else
+      pre << "const " << ity << " " << cn << " = t;\n";
   }
-  root.add(out_arg + "[o] = " + result + ";");
+  for (size_t d = 0; d < vars.size(); ++d) base.emplace_back(Expr(df.sym(vars[d])));
+
+  std::vector<std::string> results;
+  for (int e = 0; e < E; ++e) {
+    std::vector<Expr> coords = base;
+    if (!coords.empty()) coords.back() = (Expr(q) * Expr(E) + Expr(e)).simplify();
+    if (n.op == Operation::write) {
+      const auto &in = n.inputs.at(0);
+      std::map<int, Expr> subst;
+      for (size_t i = 0; i < vars.size(); ++i) subst[df.sym(vars[i]).id()] = coords[i];
+      results.push_back(I.value(&root, in.src, I.input_coords(n, in, subst)));
+    } else {
+      results.push_back(I.compute(&root, n, coords));
+    }
+  }
+  if (E == 1) {
+    root.add(out_arg + "[o] = " + results[0] + ";");
+  } else {
+    std::string vt = E == 4 ? "lt_f4" : "lt_f2";
+    std::string lit = "(" + vt + "){";
+    for (int e = 0; e < E; ++e) lit += (e ? ", " : "") + results[e];
+    lit += "}";
+    root.add("__builtin_nontemporal_store(" + lit + ", (" + vt + "*)(" + out_arg + ") + o);");
+  }
 
   std::stringstream ss;
   ss << "extern \"C\" __global__ void __launch_bounds__(" << block << ") " << name << "("

@@ -87,10 +87,52 @@ def build_reference(force=False, jobs=8):
 
 def build_all(force=False):
     out = [build_oracle(force)]
-    r = build_reference(force)
-    if r:
-        out.append(r)
+    for fn in (build_reference, build_reference_plugin):
+        r = fn(force)
+        if r:
+            out.append(r)
     return out
+
+
+
+PUB_LIB = os.path.join(REF_OUT, "libloop_tool_pub.so")
+PLUGIN_LIB = os.path.join(REF_OUT, "libloop_tool_hip.so")
+
+
+def build_reference_plugin(force=False, jobs=8):
+    """Drop-in check: the reference library with default symbol visibility
+    (as its own CMake build produces it) plus this repository's hip backend
+    sources compiled against the *reference* headers into an out-of-tree
+    plugin, loaded by the reference's loop_tool::loadLibrary."""
+    if not os.path.isdir(os.path.join(REF, "src")):
+        return None
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    objdir = os.path.join(REF_OUT, "obj_pub")
+    os.makedirs(objdir, exist_ok=True)
+    base = ["-std=c++17", "-O2", "-fPIC", "-I" + os.path.join(REF, "include"),
+            "-I" + os.path.join(ROOT, "include"), "-include", "chrono", "-w"]
+    core = [os.path.join(REF, s) for s in REF_SOURCES] + [
+        os.path.join(ROOT, "loop_tool_amd", "csrc", "capi.cpp")]
+    hipdir = os.path.join(ROOT, "loop_tool_amd", "csrc", "hip")
+    plugin = [os.path.join(hipdir, f) for f in sorted(os.listdir(hipdir)) if f.endswith(".cpp")]
+    hdrs = [os.path.join(hipdir, f) for f in os.listdir(hipdir) if f.endswith(".h")]
+    todo, core_objs, plugin_objs = [], [], []
+    for s in core + plugin:
+        obj = os.path.join(objdir, ("hip_" if s in plugin else "") + os.path.basename(s)[:-4] + ".o")
+        (plugin_objs if s in plugin else core_objs).append(obj)
+        extra = ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(rocm, "include")] if s in plugin else []
+        if force or _stale(obj, [s] + hdrs):
+            todo.append(["g++"] + base + extra + ["-c", s, "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, todo))
+    if todo or force or not os.path.exists(PUB_LIB):
+        _run(["g++", "-shared", "-o", PUB_LIB] + core_objs + ["-ldl"])
+    if todo or force or not os.path.exists(PLUGIN_LIB):
+        _run(["g++", "-shared", "-o", PLUGIN_LIB] + plugin_objs +
+             ["-L" + REF_OUT, "-lloop_tool_pub", "-Wl,-rpath,$ORIGIN",
+              "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-lhiprtc",
+              "-Wl,-rpath," + os.path.join(rocm, "lib")])
+    return PLUGIN_LIB
 
 
 if __name__ == "__main__":

@@ -44,4 +44,4 @@ def test_split_schedule_lowers_with_tails(lt):
     t = t.split(t.children(t.roots[0])[0], 10)  # K loop: 1 x 10 + 6 tail
     t = t.annotate(t.roots[0], "hip:plain")
     src = lt.compile(t, "hip").dump()
-    assert "for (long" in src
+    assert "for (int " in src or "for (long " in src
