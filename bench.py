@@ -1,6 +1,7 @@
 """Benchmark of the loop_tool hip backend on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nest softmax|matmul|conv2d|conv2d_bf16|maxpool]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--nest softmax|matmul|conv2d|conv2d_bf16|maxpool|copy|batched]
 
 Default workload: FP32 row softmax over an 8192 x 8192 tensor (max-reduce,
 subtract, exp, sum-reduce, divide over the columns), written with the lazy
@@ -123,12 +124,107 @@ def cpu_baseline(name, inputs_np, shapes, seconds, bytes_per_step, flops_per_ste
                        "cc -O3 -std=c99)" % (sample, reps, el))
 
 
+def batched_cpu_baseline(dims, seconds):
+    """The reference C backend on a sample of the batch's GEMMs (fp32 matmul
+    nests; compile time excluded), as TFLOP/s."""
+    from loop_tool_amd import capi, programs as P
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libloop_tool_ref.so")
+    kind = "reference"
+    if os.path.exists(ref_path):
+        L = capi.Library(ref_path)
+    else:
+        import loop_tool_amd as lt_mod
+        lt_mod.load_lib(os.path.join(ROOT, "oracle", "liblt_oracle.so"))
+        L, kind = lt_mod.lib, "port"
+    flops, el, done = 0, 0.0, 0
+    for i in range(len(dims)):
+        M, N, K = dims[i]
+        out, sh = P.matmul(L, M, N, K)
+        cc = L.compile(out.loop_tree, "cpp")
+        a, b = P.uniform(1, sh[0]), P.uniform(2, sh[1])
+        y = np.zeros(M * N, np.float32)
+        t0 = time.perf_counter()
+        cc.run([a, b, y])
+        el += time.perf_counter() - t0
+        flops += 2 * M * N * K
+        done += 1
+        if el >= seconds:
+            break
+    return dict(value=round(flops / el / 1e12, 6), unit="TFLOP/s", cores=1, kind=kind,
+                sample="the first %d of the %d GEMMs, fp32, %.1f s, single thread "
+                       "(reference C backend, cc -O3 -std=c99)" % (done, len(dims), el))
+
+
+def bench_batched(args):
+    """512 independent matmul nests of uneven sizes (M, N, K log-uniform in
+    [32, 2048], multiples of 16), bf16 operands, fp32 accumulation: one
+    launch per GEMM (plain) against the autotuned grouped persistent launch
+    replayed from a hipGraph.  A step is one pass over the whole batch."""
+    from loop_tool_amd import batched as B, hip
+    hip.init()
+    dims = B.shapes(n=512, seed=6)
+    stream = hip.Stream()
+    b = B.Batch(dims, dtype="bf16", seed=6, stream=stream.h)
+    flops = b.flops()
+    # correctness spot check on a few GEMMs (plain launch)
+    b.run_plain(B.DEFAULT_CFG)
+    stream.synchronize()
+    errs = [b.check(i) for i in (0, 1, 2, 3)]
+    plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
+    cfg, report = B.tune_buckets(b, iters=3)
+    grouped_ms = B.time_ms(lambda: b.run_grouped(256), iters=10, stream=stream.h)
+    b.capture(256)
+    for _ in range(args.warmup):
+        b.replay()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.replay()
+    stream.synchronize()
+    el = time.perf_counter() - t0
+    # the dominant (only) kernel of a replay: HIP events on its stream
+    kern_ms = B.time_ms(b.replay, iters=max(args.steps, 20), stream=stream.h)
+    errs += [b.check(i) for i in (0, 1, 2, 3)]
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    line = {
+        "metric": "TFLOP/s (algorithmic flops / time), batched uneven GEMMs, hip backend",
+        "value": round(flops * args.steps / el / 1e12, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (SplitMix64-seeded sizes and operands)",
+        "config": dict(workload="512 independent matmul nests, M,N,K log-uniform in "
+                                "[32, 2048] (multiples of 16), bf16 operands, fp32 accumulate",
+                       gemms=len(dims), algorithmic_flops=flops, work_items=b.num_items(),
+                       plain_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
+                       graph_ms=round(kern_ms, 5),
+                       speedup_vs_plain=round(plain_ms / kern_ms, 3),
+                       tuned_tiles=report,
+                       max_err_vs_f64=max(e for e, _ in errs),
+                       max_err_vs_f64_of_bf16_operands=max(e for _, e in errs)),
+        "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=MFMA_PEAK_TFLOPS["bf16"],
+                         unit="TFLOP/s", frac=round(achieved / MFMA_PEAK_TFLOPS["bf16"], 4),
+                         traffic=None),
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = batched_cpu_baseline(dims, args.cpu_seconds)
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has a single MI355X; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.nest == "batched":
+        return bench_batched(args)
     import loop_tool_amd as lt_mod
     from loop_tool_amd import hip
 

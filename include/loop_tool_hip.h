@@ -42,6 +42,33 @@ int lth_run_on(const ltc_compiled *cc, void *const *io, int n_io, void *const *s
 int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int iters,
              float *ms);
 
+/* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
+ * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
+ * transposed Bt[N][K]; tile configurations 0..4 are 64x64, 128x64, 64x128,
+ * 128x128, 256x128 (rows x cols of C). */
+typedef struct lth_batch lth_batch;
+const char *lth_batch_last_error(void);
+/* dtype: 0 bf16, 1 fp16; K must be a multiple of 8 */
+lth_batch *lth_batch_create(int n, const int *M, const int *N, const int *K, int dtype);
+int lth_batch_set_operands(lth_batch *b, void *const *A, void *const *Bt, void *const *C);
+/* tile configuration per GEMM (as chosen per shape bucket by the autotuner) */
+int lth_batch_set_configs(lth_batch *b, const int *cfg);
+/* number of output tiles in the grouped work list */
+int lth_batch_num_items(lth_batch *b);
+/* one launch per GEMM with tile config `cfg` (-1: each GEMM's own config) */
+int lth_batch_run_plain(lth_batch *b, int cfg, void *stream);
+/* one persistent launch of `grid` workgroups over every tile of the batch */
+int lth_batch_run_grouped(lth_batch *b, int grid, void *stream);
+/* capture the grouped launch into a hipGraph on `stream`, replay it later */
+int lth_batch_capture(lth_batch *b, int grid, void *stream);
+int lth_batch_replay(lth_batch *b, void *stream);
+void lth_batch_free(lth_batch *b);
+/* float32 [rows][cols] -> 16-bit operand (transposed to [cols][rows] if
+ * transpose != 0), on `stream` */
+int lth_convert(const float *src, void *dst, int rows, int cols, int transpose, int dtype,
+                void *stream);
+int lth_tile_config(int cfg, int *bm, int *bn);
+
 #pragma GCC visibility pop
 
 #ifdef __cplusplus

@@ -101,8 +101,14 @@ def build_kernels(force=False):
         deps = [src] + [os.path.join(kdir, h) for h in os.listdir(kdir) if h.endswith(".h")]
         if force or not os.path.exists(out) or os.path.getmtime(out) < max(
                 os.path.getmtime(d) for d in deps):
+            bundle = out + ".bundle"
             _run([os.path.join(ROCM, "bin", "hipcc"), "--genco"] + HIPCC_FLAGS +
-                 ["-I" + kdir, "-o", out, src])
+                 ["-I" + kdir, "-o", bundle, src])
+            # keep the bare gfx950 code object (what hipModuleLoadData maps)
+            _run([os.path.join(ROCM, "llvm", "bin", "clang-offload-bundler"), "--unbundle",
+                  "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
+                  "--input=" + bundle, "--output=" + out])
+            os.remove(bundle)
         built.append(out)
     return built
 

@@ -1,0 +1,297 @@
+// Batches of independent, run-time-sized matmul nests on MI355X
+// (see include/loop_tool_hip.h, lth_batch_*).
+//
+// Operands are 16-bit (bf16 or fp16) and resident in HBM: A[M][K] and B
+// stored transposed as Bt[N][K]; outputs are float32 C[M][N].  Two ways to
+// run a batch:
+//   plain    one launch of a fixed-tile kernel per GEMM (the hip backend's
+//            per-nest behaviour, one Compiled::run per matmul);
+//   grouped  one persistent launch over a work list holding every output
+//            tile of every GEMM, each with the tile configuration chosen for
+//            its shape bucket, ordered by decreasing cost so that the last
+//            tiles to start are the cheap ones; optionally captured once into
+//            a hipGraph and replayed.
+// The kernels are hand-written HIP (csrc/kernels/gemm.hip) built by hipcc
+// into kernels/gemm.hsaco next to this library.
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "hip_backend.h"
+#include "loop_tool/error.h"
+#include "loop_tool_hip.h"
+#include "lt_hip.h"
+
+namespace {
+
+struct Desc {
+  const void *A;
+  const void *Bt;
+  float *C;
+  int M, N, K, pad;
+};
+struct Item {
+  int gemm, m0, n0, cfg;
+};
+
+const int kTileM[5] = {64, 128, 64, 128, 256};
+const int kTileN[5] = {64, 64, 128, 128, 128};
+
+void chk(int rc, const char *what) {
+  ASSERT(rc == 0) << what << ": " << lt_hip_last_error();
+}
+
+std::string library_dir() {
+  Dl_info info;
+  ASSERT(dladdr((void *)&library_dir, &info) && info.dli_fname) << "cannot locate library";
+  std::string p = info.dli_fname;
+  return p.substr(0, p.find_last_of('/'));
+}
+
+struct GemmModule {
+  lt_hip_module mod = nullptr;
+  std::vector<char> code;
+  void *fn(const std::string &name) {
+    void *f = nullptr;
+    chk(lt_hip_module_get_function(mod, name.c_str(), &f), name.c_str());
+    return f;
+  }
+};
+
+GemmModule &gemm_module() {
+  static std::mutex mu;
+  static GemmModule *m = nullptr;
+  std::lock_guard<std::mutex> g(mu);
+  if (!m) {
+    loop_tool::hip::ensure_runtime();
+    std::string path = library_dir() + "/kernels/gemm.hsaco";
+    std::ifstream f(path, std::ios::binary);
+    ASSERT(f) << "kernel library missing: " << path << " (python -m loop_tool_amd.build)";
+    auto *mm = new GemmModule();
+    mm->code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+    chk(lt_hip_module_load(mm->code.data(), &mm->mod), "load gemm.hsaco");
+    m = mm;  // kept for the process lifetime
+  }
+  return *m;
+}
+
+}  // namespace
+
+struct lth_batch {
+  int dtype = 0;  // 0 bf16, 1 f16
+  std::vector<Desc> descs;
+  std::vector<int> cfg;
+  void *d_descs = nullptr, *d_items = nullptr;
+  int n_items = 0;
+  bool dirty = true;
+  void *graph = nullptr;
+  void *graph_stream = nullptr;
+
+  const char *tname() const { return dtype == 0 ? "bf16" : "f16"; }
+
+  void upload() {
+    if (!dirty) return;
+    std::vector<Item> items;
+    std::vector<int64_t> cost;
+    for (size_t g = 0; g < descs.size(); ++g) {
+      int c = cfg[g];
+      int bm = kTileM[c], bn = kTileN[c];
+      for (int m0 = 0; m0 < descs[g].M; m0 += bm)
+        for (int n0 = 0; n0 < descs[g].N; n0 += bn) {
+          items.push_back({(int)g, m0, n0, c});
+          cost.push_back((int64_t)bm * bn * descs[g].K);
+        }
+    }
+    std::vector<size_t> order(items.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<Item> sorted;
+    for (auto i : order) sorted.push_back(items[i]);
+    if (d_descs) lt_hip_free(d_descs);
+    if (d_items) lt_hip_free(d_items);
+    chk(lt_hip_malloc(&d_descs, descs.size() * sizeof(Desc)), "descs");
+    chk(lt_hip_malloc(&d_items, std::max<size_t>(sorted.size(), 1) * sizeof(Item)), "items");
+    chk(lt_hip_memcpy(d_descs, descs.data(), descs.size() * sizeof(Desc), 0), "descs copy");
+    if (!sorted.empty())
+      chk(lt_hip_memcpy(d_items, sorted.data(), sorted.size() * sizeof(Item), 0), "items copy");
+    n_items = (int)sorted.size();
+    dirty = false;
+    if (graph) {
+      lt_hip_graph_destroy(graph);
+      graph = nullptr;
+    }
+  }
+
+  void grouped(int grid, void *stream) {
+    upload();
+    if (!n_items) return;
+    void *fn = gemm_module().fn(std::string("lt_ggemm_") + tname());
+    int g = std::max(1, std::min(grid, n_items));
+    void *a0 = d_descs, *a1 = d_items;
+    int a2 = n_items;
+    void *args[] = {&a0, &a1, &a2};
+    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, 0, stream, args), "grouped launch");
+  }
+
+  void plain(int cfg_all, void *stream) {
+    for (size_t i = 0; i < descs.size(); ++i) {
+      int c = cfg_all >= 0 ? cfg_all : cfg[i];
+      auto &d = descs[i];
+      std::string name = std::string("lt_gemm_") + tname() + "_" + std::to_string(kTileM[c]) +
+                         "x" + std::to_string(kTileN[c]);
+      void *fn = gemm_module().fn(name);
+      const void *A = d.A, *Bt = d.Bt;
+      float *C = d.C;
+      int M = d.M, N = d.N, K = d.K;
+      void *args[] = {&A, &Bt, &C, &M, &N, &K};
+      unsigned gx = (N + kTileN[c] - 1) / kTileN[c], gy = (M + kTileM[c] - 1) / kTileM[c];
+      chk(lt_hip_launch(fn, gx, gy, 1, 256, 1, 1, 0, stream, args), "plain launch");
+    }
+  }
+
+  ~lth_batch() {
+    if (loop_tool::hip::runtime_exiting()) return;
+    if (graph) lt_hip_graph_destroy(graph);
+    if (d_descs) lt_hip_free(d_descs);
+    if (d_items) lt_hip_free(d_items);
+  }
+};
+
+namespace {
+thread_local std::string g_batch_err;
+template <typename F>
+int guarded(F f) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    g_batch_err = e.what();
+  }
+  return -1;
+}
+}  // namespace
+
+extern "C" {
+
+const char *lth_batch_last_error(void) { return g_batch_err.c_str(); }
+
+lth_batch *lth_batch_create(int n, const int *M, const int *N, const int *K, int dtype) {
+  try {
+    ASSERT(n > 0) << "empty batch";
+    ASSERT(dtype == 0 || dtype == 1) << "dtype must be 0 (bf16) or 1 (f16)";
+    auto *b = new lth_batch();
+    b->dtype = dtype;
+    for (int i = 0; i < n; ++i) {
+      ASSERT(M[i] > 0 && N[i] > 0 && K[i] > 0 && K[i] % 8 == 0)
+          << "GEMM " << i << ": sizes must be positive and K a multiple of 8";
+      b->descs.push_back({nullptr, nullptr, nullptr, M[i], N[i], K[i], 0});
+    }
+    b->cfg.assign(n, 3);
+    return b;
+  } catch (const std::exception &e) {
+    g_batch_err = e.what();
+    return nullptr;
+  }
+}
+
+int lth_batch_set_operands(lth_batch *b, void *const *A, void *const *Bt, void *const *C) {
+  return guarded([&] {
+    for (size_t i = 0; i < b->descs.size(); ++i) {
+      b->descs[i].A = A[i];
+      b->descs[i].Bt = Bt[i];
+      b->descs[i].C = (float *)C[i];
+    }
+    b->dirty = true;
+    return 0;
+  });
+}
+
+int lth_batch_set_configs(lth_batch *b, const int *cfg) {
+  return guarded([&] {
+    for (size_t i = 0; i < b->descs.size(); ++i) {
+      ASSERT(cfg[i] >= 0 && cfg[i] < 5) << "tile config out of range";
+      b->cfg[i] = cfg[i];
+    }
+    b->dirty = true;
+    return 0;
+  });
+}
+
+int lth_batch_num_items(lth_batch *b) {
+  return guarded([&] {
+    b->upload();
+    return b->n_items;
+  });
+}
+
+int lth_batch_run_plain(lth_batch *b, int cfg, void *stream) {
+  return guarded([&] {
+    ASSERT(cfg >= -1 && cfg < 5) << "tile config out of range";
+    b->plain(cfg, stream);
+    return 0;
+  });
+}
+
+int lth_batch_run_grouped(lth_batch *b, int grid, void *stream) {
+  return guarded([&] {
+    b->grouped(grid, stream);
+    return 0;
+  });
+}
+
+int lth_batch_capture(lth_batch *b, int grid, void *stream) {
+  return guarded([&] {
+    ASSERT(stream) << "graph capture needs a created stream (not the null stream)";
+    b->upload();
+    gemm_module();  // load outside capture
+    if (b->graph) {
+      lt_hip_graph_destroy(b->graph);
+      b->graph = nullptr;
+    }
+    chk(lt_hip_graph_begin_capture(stream), "begin capture");
+    b->grouped(grid, stream);
+    chk(lt_hip_graph_end_capture(stream, &b->graph), "end capture");
+    return 0;
+  });
+}
+
+int lth_batch_replay(lth_batch *b, void *stream) {
+  return guarded([&] {
+    ASSERT(b->graph && !b->dirty) << "capture the batch first (lth_batch_capture)";
+    chk(lt_hip_graph_launch(b->graph, stream), "graph launch");
+    return 0;
+  });
+}
+
+void lth_batch_free(lth_batch *b) { delete b; }
+
+int lth_convert(const float *src, void *dst, int rows, int cols, int transpose, int dtype,
+                void *stream) {
+  return guarded([&] {
+    ASSERT(dtype == 0 || dtype == 1) << "dtype must be 0 (bf16) or 1 (f16)";
+    void *fn = gemm_module().fn(dtype == 0 ? "lt_cvt_bf16" : "lt_cvt_f16");
+    const float *s = src;
+    void *d = dst;
+    int r = rows, c = cols, t = transpose;
+    void *args[] = {&s, &d, &r, &c, &t};
+    long n = (long)rows * cols;
+    chk(lt_hip_launch(fn, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, stream, args),
+        "convert launch");
+    return 0;
+  });
+}
+
+int lth_tile_config(int cfg, int *bm, int *bn) {
+  if (cfg < 0 || cfg >= 5) return -1;
+  *bm = kTileM[cfg];
+  *bn = kTileN[cfg];
+  return 0;
+}
+
+}  // extern "C"

@@ -41,6 +41,7 @@ def ref():
 def hip_dev():
     import loop_tool_amd.hip as hip
     if not hip.available():
-        pytest.skip("no HIP device")
+        pytest.fail("no HIP device visible: GPU tests need the MI355X and the built "
+                    "loop_tool_amd/liblt_amd.so", pytrace=False)
     hip.init()
     return hip

@@ -1,0 +1,94 @@
+"""Batches of uneven, run-time-sized matmul nests on the GPU.
+
+Every GEMM of a batch is checked against float64 of its 16-bit-rounded
+operands (the only rounding left is the float32 accumulation order, bound
+1e-5 normwise) and against float64 of the float32 operands (bf16 operand
+rounding, bound 1e-2).  The grouped persistent launch, its hipGraph replay and
+the tuned per-bucket tile choice must agree bit-for-bit with one launch per
+GEMM of the same tile shape: every output element is reduced by one lane in
+the same K order whichever way the tiles are scheduled.
+"""
+import numpy as np
+import pytest
+
+from loop_tool_amd import programs as P
+
+pytestmark = pytest.mark.gpu
+
+EDGE_DIMS = [(32, 32, 32), (48, 80, 16), (2048, 32, 64), (33 * 16, 2000, 16 * 7),
+             (16, 16, 8), (100, 36, 24), (256, 128, 2048)]
+
+
+@pytest.fixture(scope="module")
+def bat(hip_dev):
+    from loop_tool_amd import batched
+    return batched
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_plain_every_tile_vs_f64(bat, hip_dev, dtype):
+    b = bat.Batch(EDGE_DIMS, dtype=dtype, seed=11)
+    for cfg in range(len(bat.TILES)):
+        b.run_plain(cfg)
+        hip_dev.synchronize()
+        for i, d in enumerate(EDGE_DIMS):
+            e, e16 = b.check(i)
+            print(dtype, bat.TILES[cfg], d, e, e16, flush=True)
+            assert e16 <= 1e-5, (cfg, d, e16)
+            assert e <= 1e-2, (cfg, d, e)
+
+
+def test_grouped_and_graph_match_plain(bat, hip_dev):
+    dims = bat.shapes(n=64, seed=7, lo=32, hi=1024)
+    b = bat.Batch(dims, dtype="bf16", seed=3)
+    rng = np.random.RandomState(0)
+    cfg = [int(c) for c in rng.randint(0, len(bat.TILES), len(dims))]
+    b.set_configs(cfg)
+    b.run_plain(-1)
+    hip_dev.synchronize()
+    want = [b.result(i) for i in range(len(dims))]
+    for c in b.C:
+        c.fill_zero()
+    b.run_grouped(grid=97)
+    hip_dev.synchronize()
+    for i in range(len(dims)):
+        assert np.array_equal(b.result(i), want[i]), ("grouped", i, dims[i], cfg[i])
+    for c in b.C:
+        c.fill_zero()
+    b.capture(grid=256)
+    for rep in range(3):
+        b.replay()
+        hip_dev.synchronize()
+        for i in range(len(dims)):
+            assert np.array_equal(b.result(i), want[i]), ("replay", rep, i)
+        for c in b.C:  # eager work between replays
+            c.fill_zero()
+    # reconfiguring invalidates the captured graph
+    b.set_configs([0] * len(dims))
+    with pytest.raises(Exception):
+        b.replay()
+
+
+def test_tuned_batch_vs_f64(bat, hip_dev, lt, tmp_path):
+    from loop_tool_amd.autotune import TunedStore
+    dims = bat.shapes(n=48, seed=5, lo=32, hi=512)
+    b = bat.Batch(dims, dtype="bf16", seed=9)
+    store = TunedStore(str(tmp_path / "tuned.jsonl"))
+    cfg, report = bat.tune_buckets(b, iters=2, store=store, L=lt)
+    assert set(cfg) <= set(range(len(bat.TILES)))
+    assert len(store.entries) == len(report)
+    # stored schedules deserialize into LoopTrees of the matmul nest
+    for e in store.entries.values():
+        t = lt.deserialize(e["ir"])
+        assert "tile=" in t.dump()
+    b.capture()
+    b.replay()
+    hip_dev.synchronize()
+    for i in range(len(dims)):
+        e, e16 = b.check(i)
+        assert e16 <= 1e-5 and e <= 1e-2, (i, dims[i], e, e16)
+
+
+def test_batch_rejects_bad_sizes(bat, hip_dev):
+    with pytest.raises(Exception, match="multiple of 8"):
+        bat.Batch([(32, 32, 12)])
