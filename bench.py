@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--nest", default="softmax")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--annotate", default="",
+                    help="hip backend knobs for the nest, e.g. 'tile=128x256'")
     return ap.parse_args()
 
 
@@ -232,6 +234,9 @@ def main():
     hip.init()
     out, shapes, gen, cfg, bound, dtype, backend = workload(L, args.nest)
     lt = out.loop_tree
+    if args.annotate:
+        lt = lt.annotate(lt.roots[0], args.annotate)
+        cfg = dict(cfg, knobs=args.annotate)
     cc = L.compile(lt, backend)
     inputs_np = gen()
     n_out = lt.io_numel()[len(inputs_np)]
@@ -250,8 +255,12 @@ def main():
     if args.nest == "softmax":
         ref = P.reference_output("softmax", [inputs_np[0][:64]]).ravel()
         err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
+    elif args.nest == "copy":
+        err = float(np.abs(y - inputs_np[0].ravel()).max())
     else:
-        err = None
+        name = {"conv2d_bf16": "conv2d", "maxpool": "maxpool2d"}.get(args.nest, args.nest)
+        ref = P.reference_output(name, inputs_np).ravel()
+        err = float(np.abs(y - ref).max() / np.abs(ref).max())
 
     for _ in range(args.warmup):
         cc.run([b.ptr for b in io], sync=False)
@@ -263,12 +272,15 @@ def main():
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
 
-    # dominant kernel: launch duration from HIP events on its own stream
-    kern_ms = hip.time_ms(cc, io, warmup=3, iters=max(args.steps, 20))
+    # dominant kernel: its own launches timed with HIP events on its stream
+    key = "bytes" if bound == "hbm" else "flops"
+    dom = max(range(len(st)), key=lambda i: st[i][key])
+    kern_ms = hip.time_stage_ms(cc, io, dom, warmup=3, iters=max(args.steps, 20))
+    plan_ms = hip.time_ms(cc, io, warmup=3, iters=max(args.steps, 20))
     if bound == "hbm":
         value = bytes_step * args.steps / el / 1e9
         unit = "GB/s"
-        achieved = bytes_step / (kern_ms * 1e-3) / 1e9
+        achieved = st[dom]["bytes"] / (kern_ms * 1e-3) / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=MEASURED_TRAFFIC.get(args.nest))
@@ -276,7 +288,7 @@ def main():
     else:
         value = flops_step * args.steps / el / 1e12
         unit = "TFLOP/s"
-        achieved = flops_step / (kern_ms * 1e-3) / 1e12
+        achieved = st[dom]["flops"] / (kern_ms * 1e-3) / 1e12
         peak = MFMA_PEAK_TFLOPS[dtype]
         roof = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=None)
@@ -297,7 +309,8 @@ def main():
         "data": "synthetic (SplitMix64-seeded inputs)",
         "config": dict(cfg, kernels=[s["kernel"] + ":" + s["kind"] for s in st],
                        algorithmic_bytes=bytes_step, algorithmic_flops=flops_step,
-                       kernel_ms=round(kern_ms, 5), max_err_vs_f64=err),
+                       dominant_kernel=st[dom]["kernel"], kernel_ms=round(kern_ms, 5),
+                       plan_ms=round(plan_ms, 5), max_err_vs_f64=err),
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -41,6 +41,10 @@ int lth_run_on(const ltc_compiled *cc, void *const *io, int n_io, void *const *s
  * mean milliseconds per launch of the whole plan to *ms */
 int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int iters,
              float *ms);
+/* as lth_time for stage `stage` alone (after one full run, so the stage
+ * reads the scratch contents a full run leaves): mean ms per launch */
+int lth_time_stage(const ltc_compiled *cc, void *const *io, int n_io, int stage, int warmup,
+                   int iters, float *ms);
 
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
  * float32 accumulation and output).  Operands live in HBM as A[M][K] and the

@@ -30,8 +30,10 @@ CXXFLAGS = [
     "-std=c++17", "-O2", "-g0", "-fPIC", "-Wall", "-Wno-sign-compare",
     "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable",
     "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
-    "-I" + os.path.join(ROCM, "include"),
+    "-I" + os.path.join(ROCM, "include"), "-I" + os.path.join(BUILD, "gen"),
 ]
+# device headers whose text is embedded in the library for hipRTC modules
+EMBEDDED = ["igemm_tile.h"]
 # every HIP compile: gfx950 only, packed FP32 instructions disabled
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-ffp-contract=off",
@@ -64,8 +66,24 @@ def _run(cmd):
     return r.stdout
 
 
+def _embed_sources():
+    """csrc/kernels/<name>.h -> _build/gen/<name-without-.h>.inc (a raw string)."""
+    gen = os.path.join(BUILD, "gen")
+    os.makedirs(gen, exist_ok=True)
+    for h in EMBEDDED:
+        src = os.path.join(CSRC, "kernels", h)
+        text = open(src).read()
+        assert ")LTSRC\"" not in text
+        body = 'R"LTSRC(' + text + ')LTSRC"\n'
+        out = os.path.join(gen, h[:-2] + ".inc")
+        if not os.path.exists(out) or open(out).read() != body:
+            with open(out, "w") as f:
+                f.write(body)
+
+
 def build_product(force=False, jobs=8):
     os.makedirs(BUILD, exist_ok=True)
+    _embed_sources()
     newest_header = max(os.path.getmtime(h) for h in _headers())
     objs, todo = [], []
     for src in sources():

@@ -117,4 +117,32 @@ int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int 
   });
 }
 
+int lth_time_stage(const ltc_compiled *cc, void *const *io, int n_io, int stage, int warmup,
+                   int iters, float *ms) {
+  return guarded([&] {
+    auto *h = as_hip(cc);
+    std::vector<void *> mem(io, io + n_io);
+    auto chk = [](int rc) {
+      if (rc) throw std::runtime_error(lt_hip_last_error());
+    };
+    void *s = h->stream();
+    h->run(mem, false);  // every stage once: scratch holds its real contents
+    for (int i = 0; i < warmup; ++i) h->run_stage(mem, stage, s);
+    void *e0 = nullptr, *e1 = nullptr;
+    chk(lt_hip_event_create(&e0));
+    chk(lt_hip_event_create(&e1));
+    chk(lt_hip_stream_synchronize(s));
+    chk(lt_hip_event_record(e0, s));
+    for (int i = 0; i < iters; ++i) h->run_stage(mem, stage, s);
+    chk(lt_hip_event_record(e1, s));
+    chk(lt_hip_event_synchronize(e1));
+    float total = 0;
+    chk(lt_hip_event_elapsed_ms(e0, e1, &total));
+    lt_hip_event_destroy(e0);
+    lt_hip_event_destroy(e1);
+    *ms = total / (iters > 0 ? iters : 1);
+    return 0;
+  });
+}
+
 }  // extern "C"

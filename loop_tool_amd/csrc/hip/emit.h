@@ -13,7 +13,7 @@ namespace hip {
 struct BufferSpec {
   enum Kind { INPUT, OUTPUT, SCRATCH } kind;
   int index;       // position among inputs/outputs, or scratch number
-  int64_t numel;   // float elements
+  int64_t numel;   // 4-byte words (float elements; 16-bit scratch holds two each)
   IR::NodeRef node;
 };
 
@@ -45,6 +45,9 @@ struct Knobs {
   int block = 256;             // "block=N"
   int vec = 0;                 // "vec=N" elements per thread (0 = auto)
   int tile_m = 0, tile_n = 0;  // "tile=MxN" for contractions (0 = auto)
+  int bk = 0;                  // "bk=N": K tile of 16-bit contractions (32, 64)
+  bool gather = false;         // "contract=gather": 16-bit contractions without
+                               // operand staging
 };
 
 // Builds the whole launch plan for a LoopTree.

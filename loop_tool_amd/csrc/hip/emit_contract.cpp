@@ -27,6 +27,7 @@
 #include <set>
 #include <sstream>
 
+#include "contract.h"
 #include "emit_internal.h"
 
 namespace loop_tool {
@@ -68,49 +69,6 @@ bool affine(const Expr &e0, std::map<int, int64_t> &coef, int64_t &c) {
   for (auto it = coef.begin(); it != coef.end();)
     it = it->second == 0 ? coef.erase(it) : std::next(it);
   return ok;
-}
-
-struct Operand {
-  int input_index = -1;
-  std::vector<int64_t> sizes;                 // per dim
-  std::vector<int64_t> strides;               // per dim (row-major)
-  std::vector<std::map<int, int64_t>> coef;   // per dim: var sym id -> coefficient
-  std::vector<int64_t> cst;                   // per dim constant
-  std::vector<int> pred_dims;                 // dims that may leave [0, size)
-};
-
-struct VarInfo {
-  int sym;
-  int64_t size;
-  std::string name;
-};
-
-std::string decompose(const std::string &lin, const std::vector<VarInfo> &vars,
-                      const std::string &indent) {
-  // row-major: the last var is the fastest
-  std::stringstream s;
-  if (vars.empty()) return "";
-  s << indent << "{ long t_ = " << lin << ";\n";
-  for (int d = (int)vars.size() - 1; d >= 0; --d) {
-    if (d > 0)
-      s << indent << "  " << vars[d].name << " = t_ % " << vars[d].size << "L; t_ /= "
-        << vars[d].size << "L;\n";
-    else
-      s << indent << "  " << vars[d].name << " = t_;\n";
-  }
-  s << indent << "}\n";
-  return s.str();
-}
-
-std::string lin_expr(const std::map<int, int64_t> &coef, const std::vector<VarInfo> &vars) {
-  std::string out;
-  for (const auto &v : vars) {
-    auto it = coef.find(v.sym);
-    if (it == coef.end() || it->second == 0) continue;
-    out += (out.empty() ? "" : " + ") + std::string("(") + std::to_string(it->second) +
-           "L * " + v.name + ")";
-  }
-  return out.empty() ? "0L" : out;
 }
 
 }  // namespace
@@ -166,6 +124,8 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
       op.cst.push_back(c);
       auto r = expr_range(in.coords[d], extents);
       if (!r.ok) return false;
+      op.lo.push_back(r.lo);
+      op.hi.push_back(r.hi);
       if (r.lo < 0 || r.hi > op.sizes[d] - 1) op.pred_dims.push_back((int)d);
     }
   }
@@ -217,6 +177,29 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
   for (auto &v : J) ND *= v.size;
   for (auto &v : K) KD *= v.size;
   for (auto &v : B) BD *= v.size;
+
+  // 16-bit operands: staged operands and the implicit-GEMM tile (emit_igemm.cpp)
+  if ((knobs.bf16 || knobs.f16) && !knobs.gather) {
+    Contraction c;
+    c.ops[0] = ops[0];
+    c.ops[1] = ops[1];
+    c.po = po;
+    c.qo = qo;
+    c.I = I;
+    c.J = J;
+    c.B = B;
+    c.K = K;
+    c.MD = MD;
+    c.ND = ND;
+    c.KD = KD;
+    c.BD = BD;
+    int64_t st = 1;
+    for (int d = (int)W.vars.size() - 1; d >= 0; --d) {
+      c.ostride[df.sym(W.vars[d]).id()] = st;
+      st *= df.size(W.vars[d]);
+    }
+    if (try_contract_staged(df, c, knobs, plan)) return true;
+  }
 
   // ---- tile configuration ----
   const bool bf16 = knobs.bf16, f16 = knobs.f16 && !knobs.bf16;

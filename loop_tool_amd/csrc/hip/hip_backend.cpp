@@ -20,6 +20,9 @@
 //   block=N        threads per workgroup of pointwise stages
 //   vec=N          elements per lane for row reductions
 //   tile=MxN       workgroup tile of contractions
+//   bk=N           K tile (32 or 64) of 16-bit contractions
+//   contract=gather  16-bit contractions gather their operands in the tile
+//                  loader instead of staging them
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -162,6 +165,8 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       if (tok == "hip:plain") k.force_plain = true;
       else if (tok == "mfma_bf16") k.bf16 = true;
       else if (tok == "mfma_f16") k.f16 = true;
+      else if (tok == "contract=gather") k.gather = true;
+      else if (tok.rfind("bk=", 0) == 0) k.bk = std::stoi(tok.substr(3));
       else if (tok.rfind("block=", 0) == 0) k.block = std::stoi(tok.substr(6));
       else if (tok.rfind("vec=", 0) == 0) k.vec = std::stoi(tok.substr(4));
       else if (tok.rfind("tile=", 0) == 0) {
@@ -234,8 +239,10 @@ void HipCompiled::load() const {
   loaded_ = true;
 }
 
-void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream) const {
-  for (size_t k = 0; k < plan_.stages.size(); ++k) {
+void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream, int first,
+                         int last) const {
+  if (last < 0) last = (int)plan_.stages.size() - 1;
+  for (int k = first; k <= last; ++k) {
     const auto &s = plan_.stages[k];
     std::vector<void *> ptrs;
     for (auto b : s.buffers) ptrs.push_back(device_buffers.at(b));
@@ -325,6 +332,14 @@ void HipCompiled::run_on(const std::vector<void *> &device_io,
     }
   }
   launch(bufs, stream);
+}
+
+void HipCompiled::run_stage(const std::vector<void *> &device_io, int k, void *stream) const {
+  std::lock_guard<std::mutex> g(mu_);
+  load();
+  ASSERT(k >= 0 && k < (int)plan_.stages.size()) << "no stage " << k;
+  ASSERT(device_io.size() >= num_io()) << "expected " << num_io() << " device buffers";
+  launch(device_buffers(device_io), stream, k, k);
 }
 
 std::vector<int64_t> HipCompiled::scratch_sizes() const {

@@ -36,6 +36,9 @@ class HipCompiled : public Compiled {
   void run_on(const std::vector<void *> &device_io, const std::vector<void *> &scratch,
               void *stream) const;
   std::vector<int64_t> scratch_sizes() const;
+  // Launch only stage k of the plan on `stream` with device buffers (inputs
+  // then outputs) and the Compiled's own scratch (timing of one kernel).
+  void run_stage(const std::vector<void *> &device_io, int k, void *stream) const;
 
   const Plan &plan() const { return plan_; }
   int64_t algorithmic_bytes() const { return algorithmic_bytes_; }
@@ -49,7 +52,8 @@ class HipCompiled : public Compiled {
 
  private:
   void load() const;
-  void launch(const std::vector<void *> &device_buffers, void *stream) const;
+  void launch(const std::vector<void *> &device_buffers, void *stream, int first = 0,
+              int last = -1) const;
   std::vector<void *> device_buffers(const std::vector<void *> &io) const;
 
   Plan plan_;

@@ -183,6 +183,8 @@ _sig("lth_scratch_sizes", C.c_int, _P, C.POINTER(C.c_int64), C.c_int)
 _sig("lth_run_on", C.c_int, _P, C.POINTER(_P), C.c_int, C.POINTER(_P), C.c_int, _P)
 _sig("lth_time", C.c_int, _P, C.POINTER(_P), C.c_int, C.c_int, C.c_int,
      C.POINTER(C.c_float))
+_sig("lth_time_stage", C.c_int, _P, C.POINTER(_P), C.c_int, C.c_int, C.c_int, C.c_int,
+     C.POINTER(C.c_float))
 
 
 def _lth(rc):
@@ -228,4 +230,13 @@ def run_on(compiled, io, scratch=(), stream=None):
 def time_ms(compiled, io, warmup=3, iters=20):
     ms = C.c_float()
     _lth(_L.lth_time(compiled.h, _ptrs(io), len(io), warmup, iters, C.byref(ms)))
+    return ms.value
+
+
+def time_stage_ms(compiled, io, stage, warmup=3, iters=20):
+    """Mean ms of one launch of plan stage `stage` alone (HIP events on the
+    Compiled's stream)."""
+    ms = C.c_float()
+    _lth(_L.lth_time_stage(compiled.h, _ptrs(io), len(io), stage, warmup, iters,
+                           C.byref(ms)))
     return ms.value

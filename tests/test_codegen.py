@@ -12,6 +12,8 @@ CASES = [
     ("matmul", dict(M=37, N=53, K=29), "hip_bf16", "contract_bf16"),
     ("conv2d", dict(N=32, Ci=128, Co=128, H=56, W=56), "hip", "contract_f32"),
     ("conv2d", dict(N=32, Ci=128, Co=128, H=56, W=56), "hip_bf16", "contract_bf16"),
+    ("conv2d", dict(N=2, Ci=16, Co=72, H=15, W=15, stride=2), "hip_bf16", "contract_bf16"),
+    ("matmul", dict(M=300, N=200, K=704), "hip_bf16", "contract_bf16"),
     ("maxpool2d", dict(N=2, C=3, H=16, W=16), "hip", "pointwise"),
     ("concat2d", dict(B=3, M0=5, M1=4), "hip", "pointwise"),
     ("pad1d", dict(N=11, pre=2, post=3), "hip", "pointwise"),
@@ -45,3 +47,25 @@ def test_split_schedule_lowers_with_tails(lt):
     t = t.annotate(t.roots[0], "hip:plain")
     src = lt.compile(t, "hip").dump()
     assert "for (int " in src or "for (long " in src
+
+
+@pytest.mark.parametrize("tile", ["tile=64x64", "tile=128x64", "tile=64x128", "tile=128x128",
+                                  "tile=128x256", "tile=256x128", "tile=64x256",
+                                  "tile=64x64 bk=64", "tile=128x256 bk=64"])
+def test_staged_contraction_plans(lt, tile):
+    """16-bit conv: two staging stages (zero-haloed NHWC input, K-innermost
+    weights) then the implicit-GEMM tile, at every tile shape."""
+    out, _ = P.conv2d(lt, 2, 16, 72, 15, 15, stride=2)
+    t = out.loop_tree
+    t = t.annotate(t.roots[0], tile)
+    cc = lt.compile(t, "hip_bf16")
+    assert cc.string_property("kinds") == "stage,stage,contract_bf16"
+    bk = "64" if "bk=64" in tile else "32"
+    assert "staged P" in cc.dump() and tile.split()[0][5:] + "x" + bk in cc.dump()
+
+
+def test_staged_layout_of_conv_operands(lt):
+    out, _ = P.conv2d(lt, 32, 128, 128, 56, 56)
+    d = lt.compile(out.loop_tree, "hip_bf16").dump()
+    # X'[32][58][58][128] and W'[128][3][3][128], K = 9 x 128
+    assert "K=9x128" in d and "staged P 147456 and Q 13778944 elements" in d

@@ -85,7 +85,14 @@ CONTRACT = [
     ("matmul", dict(M=300, N=200, K=700)),
     ("conv2d", dict(N=2, Ci=8, Co=16, H=12, W=12)),
     ("conv2d", dict(N=3, Ci=32, Co=40, H=14, W=14)),
+    ("matmul", dict(M=300, N=200, K=704)),
+    ("conv2d", dict(N=2, Ci=16, Co=72, H=15, W=15, stride=2)),
+    ("conv2d", dict(N=4, Ci=64, Co=128, H=28, W=28)),
 ]
+
+
+def ref_kw(name, kw):
+    return {k: kw[k] for k in ("pad", "stride") if name == "conv2d" and k in kw}
 
 
 def contract_inputs(name, shapes):
@@ -104,7 +111,7 @@ def test_contraction_f32_vs_oracle(lt, hip_dev, name, kw):
     y_ref, _ = run_backend(lt, out, shapes, inputs, "cpp")
     y_hip, cc = run_backend(lt, out, shapes, inputs, "hip")
     assert cc.string_property("kinds") == "contract_f32"
-    f64 = P.reference_output(name, inputs).ravel()
+    f64 = P.reference_output(name, inputs, **ref_kw(name, kw)).ravel()
     print(name, kw, "oracle-vs-f64 %.3g hip-vs-f64 %.3g hip-vs-oracle %.3g" % (
         normwise(y_ref, f64), normwise(y_hip, f64), normwise(y_hip, y_ref)))
     assert normwise(y_hip, y_ref) <= 1e-5
@@ -118,15 +125,47 @@ def test_contraction_bf16_vs_f64(lt, hip_dev, name, kw):
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     y_hip, cc = run_backend(lt, out, shapes, inputs, "hip_bf16")
-    assert cc.string_property("kinds") == "contract_bf16"
-    f64 = P.reference_output(name, inputs).ravel()
+    assert cc.string_property("kinds").split(",")[-1] == "contract_bf16"
+    f64 = P.reference_output(name, inputs, **ref_kw(name, kw)).ravel()
     # round operands to bf16 (round to nearest even) and evaluate in float64
     def bf16(a):
         u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
         u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
         return u.astype(np.uint32).view(np.float32)
-    f64_bf = P.reference_output(name, [bf16(a) for a in inputs]).ravel()
+    f64_bf = P.reference_output(name, [bf16(a) for a in inputs], **ref_kw(name, kw)).ravel()
     e, e_bf = normwise(y_hip, f64), normwise(y_hip, f64_bf)
     print(name, kw, "bf16 hip-vs-f64 %.3g hip-vs-f64(bf16 operands) %.3g" % (e, e_bf))
     assert e <= 1e-2
+    assert e_bf <= 1e-5
+
+
+STAGED = CONTRACT
+
+
+@pytest.mark.parametrize("name,kw", STAGED, ids=["%s%d" % (c[0], i) for i, c in enumerate(STAGED)])
+@pytest.mark.parametrize("tile", ["", "tile=64x64", "tile=128x64", "tile=64x128", "tile=128x256",
+                                  "tile=128x256 bk=64", "tile=64x64 bk=64", "contract=gather"])
+def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
+    """Staged-operand lowering (operands permuted, zero-haloed and rounded to
+    bf16 by their own stages, then the implicit-GEMM tile) at every tile shape,
+    and the gathered lowering, against float64 of the bf16-rounded operands."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    t = out.loop_tree
+    if tile:
+        t = t.annotate(t.roots[0], tile)
+    cc = lt.compile(t, "hip_bf16")
+    kinds = cc.string_property("kinds")
+    assert kinds == ("contract_bf16" if tile == "contract=gather" else
+                     "stage,stage,contract_bf16"), kinds
+    y = np.zeros(t.io_numel()[len(inputs)], np.float32)
+    cc.run(inputs + [y])
+
+    def bf16(a):
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32)
+    f64_bf = P.reference_output(name, [bf16(a) for a in inputs], **ref_kw(name, kw)).ravel()
+    e_bf = normwise(y, f64_bf)
+    print(name, kw, tile, "hip-vs-f64(bf16 operands) %.3g" % e_bf)
     assert e_bf <= 1e-5
