@@ -90,6 +90,10 @@ int lt_hip_memset(void *dst, int value, size_t bytes);
 /* 1 if `ptr` is device (HBM) memory of this process, 0 otherwise */
 int lt_hip_is_device_pointer(const void *ptr);
 int lt_hip_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* stream-ordered allocation from the device's default memory pool, which
+ * keeps freed blocks cached for reuse (no driver call on the steady state) */
+int lt_hip_malloc_async(void **ptr, size_t bytes, void *stream);
+int lt_hip_free_async(void *ptr, void *stream);
 
 /* ---- streams, events, graphs -----------------------------------------------
  * No counterpart in the reference (it launched on the null stream and
@@ -103,6 +107,8 @@ int lt_hip_event_create(void **event);
 int lt_hip_event_destroy(void *event);
 int lt_hip_event_record(void *event, void *stream);
 int lt_hip_event_synchronize(void *event);
+/* 1 if every work recorded before the event has completed, 0 if not yet */
+int lt_hip_event_query(void *event);
 int lt_hip_event_elapsed_ms(void *start, void *stop, float *ms);
 int lt_hip_graph_begin_capture(void *stream);
 int lt_hip_graph_end_capture(void *stream, void **graph_exec);

@@ -43,7 +43,7 @@ HIPCC_FLAGS = [
 
 def sources():
     out = []
-    for sub in ("core", "frontends", "hip"):
+    for sub in ("core", "frontends", "hip", "serve"):
         d = os.path.join(CSRC, sub)
         out += [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".cpp")]
     out.append(os.path.join(CSRC, "capi.cpp"))

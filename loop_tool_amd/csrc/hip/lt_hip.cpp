@@ -4,6 +4,8 @@
 #include <hip/hip_runtime_api.h>
 #include <hip/hiprtc.h>
 
+#include <mutex>
+
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -164,6 +166,24 @@ int lt_hip_memcpy_async(void *dst, const void *src, size_t bytes, int kind, void
   return check(hipMemcpyAsync(dst, src, bytes, kind_of(kind), (hipStream_t)stream),
                "hipMemcpyAsync");
 }
+int lt_hip_malloc_async(void **ptr, size_t bytes, void *stream) {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    hipMemPool_t pool;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;  // never trim the pool at synchronisation points
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
+  });
+  return check(hipMallocAsync(ptr, bytes, (hipStream_t)stream), "hipMallocAsync");
+}
+
+int lt_hip_free_async(void *ptr, void *stream) {
+  return check(hipFreeAsync(ptr, (hipStream_t)stream), "hipFreeAsync");
+}
+
 int lt_hip_memset(void *dst, int value, size_t bytes) {
   return check(hipMemset(dst, value, bytes), "hipMemset");
 }
@@ -212,6 +232,15 @@ int lt_hip_event_record(void *event, void *stream) {
 }
 int lt_hip_event_synchronize(void *event) {
   return check(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+}
+int lt_hip_event_query(void *event) {
+  hipError_t e = hipEventQuery((hipEvent_t)event);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return check(e, "hipEventQuery");
 }
 int lt_hip_event_elapsed_ms(void *start, void *stop, float *ms) {
   return check(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),

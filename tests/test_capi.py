@@ -14,7 +14,7 @@ LIB = os.path.join(ROOT, "loop_tool_amd", "liblt_amd.so")
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b((?:ltc|lth|lt_hip)_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b((?:ltc|lth|lts|lt_hip)_[a-z0-9_]+)\s*\(", text)))
 
 
 def exported(path):
@@ -23,7 +23,8 @@ def exported(path):
     return {line.split()[-1] for line in out.splitlines() if line.strip()}
 
 
-@pytest.mark.parametrize("header", ["loop_tool_c.h", "lt_hip.h", "loop_tool_hip.h"])
+@pytest.mark.parametrize("header", ["loop_tool_c.h", "lt_hip.h", "loop_tool_hip.h",
+                                    "loop_tool_serve.h"])
 def test_product_exports_every_declared_symbol(header):
     names = declared(header)
     assert len(names) > 5
