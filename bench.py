@@ -59,6 +59,12 @@ def workload(lt, name):
         out, shapes = P.matmul(lt, 128, 128, 128)
         gen = lambda: [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
         return out, shapes, gen, dict(workload="fp32 matmul M=N=K=128"), "mfma", "f32", "hip"
+    if name.startswith("matmul_bf16"):
+        n = int(name.split("_")[-1]) if name.count("_") == 2 else 8192
+        out, shapes = P.matmul(lt, n, n, n)
+        gen = lambda: [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
+        return out, shapes, gen, dict(workload="matmul bf16 operands, fp32 accumulate, "
+                                               "M=N=K=%d" % n), "mfma", "bf16", "hip_bf16"
     if name in ("conv2d", "conv2d_bf16"):
         N, Ci, Co, H = 32, 128, 128, 56
         out, shapes = P.conv2d(lt, N, Ci, Co, H, H)
@@ -103,6 +109,13 @@ def cpu_baseline(name, inputs_np, shapes, seconds, bytes_per_step, flops_per_ste
         ins = [inputs_np[0][:rows]]
         scale = rows / shapes[0][0]
         sample = "%d of %d rows" % (rows, shapes[0][0])
+    elif name.startswith("matmul_bf16"):
+        rows = 16
+        M, K = shapes[0]
+        out, sh = P.matmul(L, rows, shapes[1][1], K)
+        ins = [inputs_np[0][:rows], inputs_np[1]]
+        scale = rows / M
+        sample = "%d of %d rows of A (fp32)" % (rows, M)
     else:
         out, sh = workload(L, name)[:2]
         ins = inputs_np
@@ -358,6 +371,9 @@ def main():
         err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
     elif args.nest == "copy":
         err = float(np.abs(y - inputs_np[0].ravel()).max())
+    elif args.nest.startswith("matmul_bf16"):
+        ref = P.reference_output("matmul", [inputs_np[0][:64], inputs_np[1]]).ravel()
+        err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
     else:
         name = {"conv2d_bf16": "conv2d", "maxpool": "maxpool2d"}.get(args.nest, args.nest)
         ref = P.reference_output(name, inputs_np).ravel()

@@ -89,13 +89,10 @@ Staged stage_layout(const Operand &op, int inner_dim, int64_t kin_padded) {
   return s;
 }
 
-// One thread per 8 staged elements along the innermost staged dim.  Lanes
-// walk the dim that is contiguous in the source (when it is not the innermost
-// staged dim) so the source reads coalesce.
-std::string staging_kernel(const std::string &name, const Operand &op, const Staged &s,
-                           int64_t kin, const char *elt, int64_t *threads) {
-  const int nd = (int)op.sizes.size();
-  const int in = s.inner_dim;
+// The staged dim that is contiguous in the source, when it is not the
+// innermost staged dim (-1 otherwise).
+int source_contiguous_dim(const Operand &op, const Staged &s) {
+  const int nd = (int)op.sizes.size(), in = s.inner_dim;
   int ld = -1;
   int64_t best = op.strides[in];
   for (int d = 0; d < nd; ++d)
@@ -103,6 +100,91 @@ std::string staging_kernel(const std::string &name, const Operand &op, const Sta
       best = op.strides[d];
       ld = d;
     }
+  return ld;
+}
+
+// Staging as a transpose through LDS, for an operand whose source-contiguous
+// dim `ld` is not the staged inner dim (a convolution input X[N][C][H][W]
+// staged as X'[N][H+2][W+2][C]): a workgroup reads a 64 (ld) x TI (inner)
+// tile with lanes along ld (coalesced reads), and writes it back with
+// TI/8 lanes covering the 16-byte chunks of one ld position (coalesced
+// writes).
+std::string staging_transpose_kernel(const std::string &name, const Operand &op,
+                                     const Staged &s, int ld, int64_t kin, const char *elt,
+                                     unsigned grid[3]) {
+  const int nd = (int)op.sizes.size(), in = s.inner_dim;
+  const int TL = 64, TI = s.ssize[in] % 64 == 0 ? 64 : 32;
+  std::vector<int> other;
+  for (int q = 0; q < nd; ++q)
+    if (s.order[q] != ld && s.order[q] != in) other.push_back(s.order[q]);
+  int64_t nz = 1;
+  for (int d : other) nz *= s.ssize[d];
+  grid[0] = (unsigned)((s.ssize[ld] + TL - 1) / TL);
+  grid[1] = (unsigned)(s.ssize[in] / TI);
+  ASSERT(nz < 65536) << "staging grid too large";
+  grid[2] = (unsigned)nz;
+  auto idx = [&](int d, const std::string &sv) {
+    return "(" + sv + " - " + std::to_string(s.padlo[d]) + "L)";
+  };
+  std::stringstream k;
+  k << "extern \"C\" __global__ void __launch_bounds__(256) " << name
+    << "(const float* __restrict__ in, " << elt << "* __restrict__ S) {\n"
+    << "  typedef " << elt << " T8 __attribute__((ext_vector_type(8)));\n"
+    << "  __shared__ float tile[" << TI << "][" << TL + 1 << "];\n"
+    << "  const int tid = threadIdx.x;\n"
+    << "  const long l0 = (long)blockIdx.x * " << TL << ", c0 = (long)blockIdx.y * " << TI
+    << ";\n  long z_ = blockIdx.z;\n";
+  for (int q = (int)other.size() - 1; q >= 0; --q) {
+    int d = other[q];
+    if (q > 0)
+      k << "  const long s" << d << " = z_ % " << s.ssize[d] << "L; z_ /= " << s.ssize[d]
+        << "L;\n";
+    else
+      k << "  const long s" << d << " = z_;\n";
+  }
+  // validity and offsets of the dims fixed for the workgroup
+  std::string ok = "true", base = "0L", dbase = "0L";
+  for (int d : other) {
+    const std::string sv = "s" + std::to_string(d);
+    if (s.ssize[d] != op.sizes[d])
+      ok += " && " + idx(d, sv) + " >= 0 && " + idx(d, sv) + " < " +
+            std::to_string(op.sizes[d]) + "L";
+    base += " + " + idx(d, sv) + " * " + std::to_string(op.strides[d]) + "L";
+    dbase += " + " + sv + " * " + std::to_string(s.sstride[d]) + "L";
+  }
+  std::string lok;
+  if (s.ssize[ld] != op.sizes[ld])
+    lok = " && " + idx(ld, "sl") + " >= 0 && " + idx(ld, "sl") + " < " +
+          std::to_string(op.sizes[ld]) + "L";
+  k << "  const bool fixed_ok = " << ok << ";\n"
+    << "  const long src0 = " << base << ", dst0 = " << dbase << ";\n"
+    << "  #pragma unroll\n  for (int q = 0; q < " << TI / 4 << "; ++q) {\n"
+    << "    const int r = (tid >> 6) + q * 4, l = tid & 63;\n"
+    << "    const long sl = l0 + l, si = c0 + r;\n"
+    << "    const bool ok = fixed_ok && sl < " << s.ssize[ld] << "L && si < " << kin << "L"
+    << lok << ";\n"
+    << "    tile[r][l] = ok ? in[src0 + " << idx(ld, "sl") << " * " << op.strides[ld]
+    << "L + si * " << op.strides[in] << "L] : 0.0f;\n  }\n"
+    << "  __syncthreads();\n"
+    << "  #pragma unroll\n  for (int q = 0; q < " << TL * TI / 8 / 256 << "; ++q) {\n"
+    << "    const int e = tid + q * 256, g = e % " << TI / 8 << ", l = e / " << TI / 8
+    << ";\n"
+    << "    const long sl = l0 + l;\n"
+    << "    if (sl >= " << s.ssize[ld] << "L) continue;\n"
+    << "    T8 v;\n    #pragma unroll\n    for (int j = 0; j < 8; ++j) v[j] = (" << elt
+    << ")tile[g * 8 + j][l];\n"
+    << "    *(T8*)(S + dst0 + sl * " << s.sstride[ld] << "L + c0 + g * 8) = v;\n  }\n}\n";
+  return k.str();
+}
+
+// One thread per 8 staged elements along the innermost staged dim.  Lanes
+// walk the dim that is contiguous in the source (when it is not the innermost
+// staged dim) so the source reads coalesce.
+std::string staging_kernel(const std::string &name, const Operand &op, const Staged &s,
+                           int64_t kin, const char *elt, int64_t *threads) {
+  const int nd = (int)op.sizes.size();
+  const int in = s.inner_dim;
+  const int ld = source_contiguous_dim(op, s);
   // thread decomposition, fastest first: (dim, extent)
   std::vector<std::pair<int, int64_t>> tord;
   if (ld >= 0) tord.push_back({ld, s.ssize[ld]});
@@ -186,34 +268,40 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     }
   const int64_t KI = c.K[kin].size;
   ASSERT(ko_n < (1LL << 31) && KI < (1LL << 31)) << "reduction too large";
-  const int BK = knobs.bk > 0 ? knobs.bk : 32;
-  if (BK != 32 && BK != 64) return false;
-  const int64_t KIP = (KI + BK - 1) / BK * BK;
-  Staged st[2] = {stage_layout(c.ops[0], dims[0], KIP), stage_layout(c.ops[1], dims[1], KIP)};
-  const Staged &SP = st[c.po], &SQ = st[c.qo];
 
-  // ---- tile: the largest tile that still gives two workgroups per CU ----
+  // ---- tile: the largest tile (fewest LDS bytes per flop) that still gives
+  // every CU a workgroup; K tile 64 where the LDS of two stages allows ----
   auto tiles = [&](int bm, int bn) {
     return ((c.MD + bm - 1) / bm) * ((c.ND + bn - 1) / bn) * c.BD;
   };
-  int BM = 64, BN = 64;
+  int BM = 64, BN = 64, BK = 32;
   if (knobs.tile_m > 0) {
     BM = knobs.tile_m;
     BN = knobs.tile_n;
+    BK = knobs.bk > 0 ? knobs.bk : 32;
   } else {
-    const int cand[][2] = {{128, 256}, {256, 128}, {128, 128}, {64, 256},
-                           {128, 64},  {64, 128},  {64, 64}};
+    const int cand[][3] = {{256, 256, 64}, {128, 256, 32}, {256, 128, 32}, {128, 128, 64},
+                           {128, 64, 32},  {64, 128, 32},  {64, 64, 32}};
     for (auto &t : cand) {
       if (t[0] > ((c.MD + 63) / 64) * 64 || t[1] > ((c.ND + 63) / 64) * 64) continue;
-      if (tiles(t[0], t[1]) >= 512) {
+      if (tiles(t[0], t[1]) >= 256) {
         BM = t[0];
         BN = t[1];
+        BK = t[2];
         break;
       }
     }
+    if (knobs.bk > 0) BK = knobs.bk;
   }
-  if (BM % 64 || BN % 64 || BM > 256 || BN > 256 || BM * BN > 256 * 128) return false;
+  if (BK != 32 && BK != 64) return false;
+  if (BM % 64 || BN % 64 || BM > 256 || BN > 256 || BM * BN > 256 * 256) return false;
+  if (2 * (BM + BN) * (BK + 8) * 2 > 160 * 1024) return false;  // LDS of a CU
+  const int64_t KIP = (KI + BK - 1) / BK * BK;
+  Staged st[2] = {stage_layout(c.ops[0], dims[0], KIP), stage_layout(c.ops[1], dims[1], KIP)};
+  const Staged &SP = st[c.po], &SQ = st[c.qo];
   const int lds = 2 * (BM + BN) * (BK + 8) * 2;
+  const bool full = c.MD % BM == 0 && c.ND % BN == 0;
+  const bool int32 = SP.total < (1LL << 31) && SQ.total < (1LL << 31);
 
   // ---- buffers: one staged (16-bit) scratch buffer per operand ----
   int sbuf[2];
@@ -230,15 +318,22 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   std::stringstream k;
   k << kernel_prelude() << igemm_tile_source() << "\n";
   for (int o = 0; o < 2; ++o) {
-    int64_t threads = 0;
     std::string name = "lt_stage" + std::to_string(o);
-    k << staging_kernel(name, c.ops[o], st[o], KI, elt, &threads);
     Stage s;
+    const int ld = source_contiguous_dim(c.ops[o], st[o]);
+    if (ld >= 0 && c.ops[o].strides[ld] == 1 && st[o].ssize[ld] >= 32 &&
+        st[o].ssize[st[o].inner_dim] % 32 == 0 &&
+        st[o].total / (st[o].ssize[ld] * st[o].ssize[st[o].inner_dim]) < 65536) {
+      k << staging_transpose_kernel(name, c.ops[o], st[o], ld, KI, elt, s.grid);
+    } else {
+      int64_t threads = 0;
+      k << staging_kernel(name, c.ops[o], st[o], KI, elt, &threads);
+      s.grid[0] = (unsigned)((threads + 255) / 256);
+    }
     s.kernel = name;
     s.kind = "stage";
     s.buffers = {c.ops[o].input_index, sbuf[o]};
     s.writes = {sbuf[o]};
-    s.grid[0] = (unsigned)((threads + 255) / 256);
     s.block[0] = 256;
     s.bytes = df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
               st[o].total * 2;
@@ -276,7 +371,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << "  g.zp = " << lin_over(SP.scoef, c.B) << ";\n"
     << "  g.zq = " << lin_over(SQ.scoef, c.B) << ";\n"
     << "  g.zo = " << lin_over(c.ostride, c.B) << ";\n"
-    << "  lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK
+    << "  lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+    << (full ? "true" : "false") << ", " << (int32 ? "int" : "long")
     << ">(g, P, Q, out, (long)blockIdx.y * " << BM
     << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem);\n}\n";
 

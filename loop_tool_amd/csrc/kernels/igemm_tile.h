@@ -62,7 +62,11 @@ __host__ __device__ constexpr int lt_igemm_lds_bytes() {
   return 2 * (BM + BN) * (BK + 8) * 2;
 }
 
-template <typename T, int BM, int BN, int BK, class G>
+// Row offsets are computed once per tile in IT (int when every staged
+// operand has fewer than 2^31 elements); FULL drops the row bounds checks
+// when the tile grid divides the problem exactly.  The next k16 step's MFMA
+// fragments are read from LDS while the current step's MFMAs run.
+template <typename T, int BM, int BN, int BK, bool FULL = false, typename IT = long, class G>
 __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ P,
                                               const T* __restrict__ Q,
                                               float* __restrict__ out, long i0, long j0,
@@ -70,7 +74,7 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
   typedef typename lt_ig_vec8<T>::type V;
   constexpr int NT = 256, LDR = BK + 8;  // LDS row (16-bit elements)
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 32, TN = WN / 32;
-  constexpr int CPR = BK / 8;                 // 16-byte chunks per row
+  constexpr int CPR = BK / 8, KK = BK / 16;   // 16-byte chunks per row, k16 steps
   constexpr int VP = BM * CPR / NT, VQ = BN * CPR / NT;
   static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
                 "tile too small for 256 lanes");
@@ -83,24 +87,26 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
   const int cc = (tid % CPR) * 8;  // this lane's k chunk within a K tile
 
   // row offsets of the rows this lane loads (fixed for the whole K loop)
-  long pb[VP], qb[VQ];
+  IT pb[VP], qb[VQ];
   bool pok[VP], qok[VQ];
 #pragma unroll
-  for (int v = 0; v < VP; ++v) pb[v] = g.p_row(i0 + (tid + v * NT) / CPR, pok[v]);
+  for (int v = 0; v < VP; ++v) pb[v] = (IT)g.p_row(i0 + (tid + v * NT) / CPR, pok[v]);
 #pragma unroll
-  for (int v = 0; v < VQ; ++v) qb[v] = g.q_row(j0 + (tid + v * NT) / CPR, qok[v]);
+  for (int v = 0; v < VQ; ++v) qb[v] = (IT)g.q_row(j0 + (tid + v * NT) / CPR, qok[v]);
 
   lt_ig_u16x8 rp[VP], rq[VQ];
   auto load = [&](int kt) {
     const int ko = kt / KIT, ki = (kt % KIT) * BK + cc;
     const bool kok = (g.KI % BK == 0) || ki < g.KI;
-    const long po = g.p_koff(ko) + ki, qo = g.q_koff(ko) + ki;
+    const IT po = (IT)g.p_koff(ko) + ki, qo = (IT)g.q_koff(ko) + ki;
 #pragma unroll
     for (int v = 0; v < VP; ++v)
-      rp[v] = (pok[v] && kok) ? *(const lt_ig_u16x8*)(P + pb[v] + po) : (lt_ig_u16x8){};
+      rp[v] = ((FULL || pok[v]) && kok) ? *(const lt_ig_u16x8*)(P + (pb[v] + po))
+                                        : (lt_ig_u16x8){};
 #pragma unroll
     for (int v = 0; v < VQ; ++v)
-      rq[v] = (qok[v] && kok) ? *(const lt_ig_u16x8*)(Q + qb[v] + qo) : (lt_ig_u16x8){};
+      rq[v] = ((FULL || qok[v]) && kok) ? *(const lt_ig_u16x8*)(Q + (qb[v] + qo))
+                                        : (lt_ig_u16x8){};
   };
   auto store = [&](int buf) {
 #pragma unroll
@@ -109,6 +115,17 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 #pragma unroll
     for (int v = 0; v < VQ; ++v)
       *(lt_ig_u16x8*)(sQ + (buf * BN + (tid + v * NT) / CPR) * LDR + cc) = rq[v];
+  };
+  V fa[2][TM], fb[2][TN];
+  auto frags = [&](int s, int buf, int kk) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[s][a] = *(const V*)(sP + (buf * BM + wm * WM + a * 32 + (lane & 31)) * LDR +
+                             kk * 16 + (lane >> 5) * 8);
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+      fb[s][b] = *(const V*)(sQ + (buf * BN + wn * WN + b * 32 + (lane & 31)) * LDR +
+                             kk * 16 + (lane >> 5) * 8);
   };
 
   lt_ig_f32x16 acc[TM][TN];
@@ -124,21 +141,15 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
   for (int kt = 0; kt < KT; ++kt) {
     const bool more = kt + 1 < KT;
     if (more) load(kt + 1);
+    frags(0, buf, 0);
 #pragma unroll
-    for (int kk = 0; kk < BK / 16; ++kk) {
-      V av[TM], bv[TN];
-#pragma unroll
-      for (int a = 0; a < TM; ++a)
-        av[a] = *(const V*)(sP + (buf * BM + wm * WM + a * 32 + (lane & 31)) * LDR +
-                            kk * 16 + (lane >> 5) * 8);
-#pragma unroll
-      for (int b = 0; b < TN; ++b)
-        bv[b] = *(const V*)(sQ + (buf * BN + wn * WN + b * 32 + (lane & 31)) * LDR +
-                            kk * 16 + (lane >> 5) * 8);
+    for (int kk = 0; kk < KK; ++kk) {
+      if (kk + 1 < KK) frags((kk + 1) & 1, buf, kk + 1);
 #pragma unroll
       for (int a = 0; a < TM; ++a)
 #pragma unroll
-        for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(av[a], bv[b], acc[a][b]);
+        for (int b = 0; b < TN; ++b)
+          acc[a][b] = lt_ig_mfma(fa[kk & 1][a], fb[kk & 1][b], acc[a][b]);
     }
     if (more) {
       buf ^= 1;
@@ -150,14 +161,15 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const long j = j0 + wn * WN + b * 32 + (lane & 31);
-    if (j >= g.ND) continue;
+    if (!FULL && j >= g.ND) continue;
     const long oj = g.out_col(j);
 #pragma unroll
     for (int a = 0; a < TM; ++a) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const long i = i0 + wm * WM + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (i < g.MD) __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
+        if (FULL || i < g.MD)
+          __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
       }
     }
   }
