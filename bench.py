@@ -26,7 +26,7 @@ MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f32": 157.3}
 # HBM bytes per launch of the dominant kernel, from FETCH_SIZE / WRITE_SIZE
 # counter runs scaled by a copy kernel of known size
 # (profiles/lt_row0_softmax_8192x8192_hbm_traffic.md)
-MEASURED_TRAFFIC = {"softmax": 537182208}
+MEASURED_TRAFFIC = {"softmax": 537468518}
 
 
 def parse():
@@ -176,35 +176,48 @@ def batched_cpu_baseline(dims, seconds):
 
 def bench_batched(args):
     """512 independent matmul nests of uneven sizes (M, N, K log-uniform in
-    [32, 2048], multiples of 16), bf16 operands, fp32 accumulation: one
-    launch per GEMM (plain) against the autotuned grouped persistent launch
-    replayed from a hipGraph.  A step is one pass over the whole batch."""
+    [32, 2048], multiples of 16), bf16 (and fp16) operands, fp32 accumulation:
+    one launch per GEMM (plain) against the autotuned grouped persistent
+    launch replayed from a hipGraph.  A step is one pass over the whole
+    batch."""
     from loop_tool_amd import batched as B, hip
     hip.init()
     dims = B.shapes(n=512, seed=6)
     stream = hip.Stream()
-    b = B.Batch(dims, dtype="bf16", seed=6, stream=stream.h)
-    flops = b.flops()
-    # correctness spot check on a few GEMMs (plain launch)
-    b.run_plain(B.DEFAULT_CFG)
-    stream.synchronize()
-    errs = [b.check(i) for i in (0, 1, 2, 3)]
-    plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
-    cfg, report = B.tune_buckets(b, iters=3)
-    grouped_ms = B.time_ms(lambda: b.run_grouped(256), iters=10, stream=stream.h)
-    b.capture(256)
-    for _ in range(args.warmup):
-        b.replay()
-    stream.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        b.replay()
-    stream.synchronize()
-    el = time.perf_counter() - t0
-    # the dominant (only) kernel of a replay: HIP events on its stream
-    kern_ms = B.time_ms(b.replay, iters=max(args.steps, 20), stream=stream.h)
-    errs += [b.check(i) for i in (0, 1, 2, 3)]
-    achieved = flops / (kern_ms * 1e-3) / 1e12
+    flops = sum(2 * m * n * k for m, n, k in dims)
+    # algorithmic HBM bytes: 16-bit operands read once, fp32 output written once
+    nbytes = sum(2 * (m * k + k * n) + 4 * m * n for m, n, k in dims)
+    out = {}
+    for dt in ("bf16", "f16"):
+        b = B.Batch(dims, dtype=dt, seed=6, stream=stream.h)
+        b.run_plain(B.DEFAULT_CFG)
+        stream.synchronize()
+        errs = [b.check(i) for i in (0, 1, 2, 3)]
+        plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
+        cfg, report = B.tune_buckets(b, iters=3)
+        grouped_ms = B.time_ms(lambda: b.run_grouped(), iters=10, stream=stream.h)
+        b.capture()
+        if dt == "bf16":
+            for _ in range(args.warmup):
+                b.replay()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                b.replay()
+            stream.synchronize()
+            el = time.perf_counter() - t0
+        # the dominant (only) kernel of a replay: HIP events on its stream
+        graph_ms = B.time_ms(b.replay, iters=max(args.steps, 20), stream=stream.h)
+        errs += [b.check(i) for i in (0, 1, 2, 3)]
+        out[dt] = dict(plain_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
+                       graph_ms=round(graph_ms, 5),
+                       speedup_vs_plain=round(plain_ms / graph_ms, 3),
+                       work_items=b.num_items(), tuning=report,
+                       max_err_vs_f64=max(e for e, _ in errs),
+                       max_err_vs_f64_of_rounded_operands=max(e for _, e in errs))
+        b.free()
+    kern_ms = out["bf16"]["graph_ms"]
+    achieved = nbytes / (kern_ms * 1e-3) / 1e9
     line = {
         "metric": "TFLOP/s (algorithmic flops / time), batched uneven GEMMs, hip backend",
         "value": round(flops * args.steps / el / 1e12, 3),
@@ -220,16 +233,12 @@ def bench_batched(args):
         "data": "synthetic (SplitMix64-seeded sizes and operands)",
         "config": dict(workload="512 independent matmul nests, M,N,K log-uniform in "
                                 "[32, 2048] (multiples of 16), bf16 operands, fp32 accumulate",
-                       gemms=len(dims), algorithmic_flops=flops, work_items=b.num_items(),
-                       plain_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
-                       graph_ms=round(kern_ms, 5),
-                       speedup_vs_plain=round(plain_ms / kern_ms, 3),
-                       tuned_tiles=report,
-                       max_err_vs_f64=max(e for e, _ in errs),
-                       max_err_vs_f64_of_bf16_operands=max(e for _, e in errs)),
-        "roofline": dict(bound="mfma", achieved=round(achieved, 3), peak=MFMA_PEAK_TFLOPS["bf16"],
-                         unit="TFLOP/s", frac=round(achieved / MFMA_PEAK_TFLOPS["bf16"], 4),
-                         traffic=None),
+                       gemms=len(dims), algorithmic_flops=flops, algorithmic_bytes=nbytes,
+                       bf16=out["bf16"], f16=out["f16"],
+                       note="the batch's arithmetic intensity (flops/bytes) is below the "
+                            "MI355X ridge point (2.5 PFLOP/s / 8 TB/s), so HBM bounds it"),
+        "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,
+                         unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None),
     }
     if not args.no_cpu:
         line["cpu_baseline"] = batched_cpu_baseline(dims, args.cpu_seconds)

@@ -48,8 +48,8 @@ int lth_time_stage(const ltc_compiled *cc, void *const *io, int n_io, int stage,
 
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
  * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
- * transposed Bt[N][K]; tile configurations 0..4 are 64x64, 128x64, 64x128,
- * 128x128, 256x128 (rows x cols of C). */
+ * transposed Bt[N][K]; tile configurations 0..5 are 64x64, 128x64, 64x128,
+ * 128x128, 256x128, 128x256 (rows x cols of C). */
 typedef struct lth_batch lth_batch;
 const char *lth_batch_last_error(void);
 /* dtype: 0 bf16, 1 fp16; K must be a multiple of 8 */

@@ -5,9 +5,10 @@ and rounded to multiples of 16 (SplitMix64 seed), 16-bit (bf16 or fp16)
 operands resident in HBM, float32 accumulation and output.
 
 Each GEMM is a loop_tool matmul nest; the autotuner picks a schedule per
-shape bucket -- the nest split by the tile (BM, BN, 32) and annotated
-`tile=BMxBN` -- by timing the bucket's GEMMs under every tile configuration,
-and stores the tuned LoopTree (IR text) per bucket.  The tuned batch then
+shape bucket -- the nest split by the tile (BM, BN, BK) and annotated
+`tile=BMxBN bk=BK` -- by timing the whole batch's grouped launch under
+candidate tile configurations, and stores the tuned LoopTree (IR text) per
+bucket.  The tuned batch then
 runs as one persistent launch over every output tile of every GEMM, captured
 once into a hipGraph; the plain form is one launch per GEMM with the default
 128x128 tile.
@@ -37,7 +38,8 @@ for _n, _a in (("lth_batch_set_operands", [_P, C.POINTER(_P), C.POINTER(_P), C.P
 _L.lth_batch_free.restype = None
 _L.lth_batch_free.argtypes = [_P]
 
-TILES = [(64, 64), (128, 64), (64, 128), (128, 128), (256, 128)]
+TILES = [(64, 64), (128, 64), (64, 128), (128, 128), (256, 128), (128, 256)]
+TILE_K = [32, 32, 32, 64, 32, 32]
 DEFAULT_CFG = 3
 
 
@@ -101,10 +103,10 @@ class Batch:
     def run_plain(self, cfg=DEFAULT_CFG):
         _chk(_L.lth_batch_run_plain(self.h, cfg, self.stream))
 
-    def run_grouped(self, grid=256):
+    def run_grouped(self, grid=512):
         _chk(_L.lth_batch_run_grouped(self.h, grid, self.stream))
 
-    def capture(self, grid=256):
+    def capture(self, grid=512):
         _chk(_L.lth_batch_capture(self.h, grid, self.stream))
 
     def replay(self):
@@ -166,44 +168,55 @@ def time_ms(fn, iters=10, stream=None):
     return e0.elapsed_ms(e1) / iters
 
 
-def tune_buckets(batch, grid=256, iters=5, store=None, L=None):
-    """Per shape bucket, time the bucket's GEMMs (as a grouped launch) under
-    each tile configuration and keep the fastest.  Returns the per-GEMM
-    configuration list and a report; stores one tuned LoopTree per bucket."""
+def tune_buckets(batch, grid=512, iters=3, store=None, L=None, rounds=1):
+    """Choose a tile configuration per shape bucket *in the context of the
+    whole batch*: start from the best uniform configuration, then, bucket by
+    bucket (most flops first), keep any configuration that makes the whole
+    grouped launch faster by more than 1%.  (Timing a bucket alone favours
+    small tiles, which only pay off when the bucket has the GPU to itself.)
+    Returns the per-GEMM configurations and a report; stores one tuned
+    LoopTree per bucket."""
     groups = {}
     for i, (m, n, k) in enumerate(batch.dims):
         groups.setdefault(bucket(m, n, k), []).append(i)
-    cfg = [DEFAULT_CFG] * len(batch.dims)
-    report = {}
-    for key, idx in sorted(groups.items()):
-        sub = Batch.__new__(Batch)  # share the operands of the bucket's GEMMs
-        sub.dims = [batch.dims[i] for i in idx]
-        sub.dtype = batch.dtype
-        sub.stream = batch.stream
-        n = len(idx)
-        arr = lambda j: (C.c_int * n)(*[d[j] for d in sub.dims])
-        sub.h = _L.lth_batch_create(n, arr(0), arr(1), arr(2), sub.dtype)
-        if not sub.h:
-            raise HipError(_L.lth_batch_last_error().decode())
-        ptrs = lambda xs: (_P * n)(*[xs[i].ptr for i in idx])
-        _chk(_L.lth_batch_set_operands(sub.h, ptrs(batch.A), ptrs(batch.Bt), ptrs(batch.C)))
-        times = {}
-        for c in range(len(TILES)):
-            sub.set_configs([c] * n)
-            times[c] = time_ms(lambda: sub.run_grouped(grid), iters, batch.stream)
-        sub.free()
-        best = min(times, key=times.get)
-        for i in idx:
-            cfg[i] = best
-        report["%dx%d" % (2 ** key[0], 2 ** key[1])] = dict(
-            gemms=n, best="%dx%d" % TILES[best],
-            ms={"%dx%d" % TILES[c]: round(t, 4) for c, t in times.items()})
+    n = len(batch.dims)
+    stream = batch.stream
+
+    def measure(cfg):
+        batch.set_configs(cfg)
+        return time_ms(lambda: batch.run_grouped(grid), iters, stream)
+
+    uniform = {c: measure([c] * n) for c in range(len(TILES))}
+    best_c = min(uniform, key=uniform.get)
+    cfg = [best_c] * n
+    best = uniform[best_c]
+    order = sorted(groups, key=lambda key: -sum(2 * batch.dims[i][0] * batch.dims[i][1]
+                                                 * batch.dims[i][2] for i in groups[key]))
+    for _ in range(rounds):
+        for key in order:
+            idx = groups[key]
+            for c in range(len(TILES)):
+                if c == cfg[idx[0]]:
+                    continue
+                trial = list(cfg)
+                for i in idx:
+                    trial[i] = c
+                t = measure(trial)
+                if t < best * 0.99:
+                    best, cfg = t, trial
+    batch.set_configs(cfg)
+    report = dict(uniform_ms={"%dx%d" % TILES[c]: round(t, 4) for c, t in uniform.items()},
+                  tuned_ms=round(best, 4), buckets={})
+    for key in order:
+        c = cfg[groups[key][0]]
+        report["buckets"]["%dx%d" % (2 ** key[0], 2 ** key[1])] = dict(
+            gemms=len(groups[key]), tile="%dx%dx%d" % (TILES[c] + (TILE_K[c],)))
         if store is not None and L is not None:
-            m, nn, k = sub.dims[0]
+            m, nn, k = batch.dims[groups[key][0]]
             store.put("matmul-bucket-%dx%d-%s" % (2 ** key[0], 2 ** key[1],
                                                   "bf16" if batch.dtype == 0 else "f16"),
-                      tuned_tree(L, m, nn, k, best), times[best], "tile=%dx%d" % TILES[best])
-    batch.set_configs(cfg)
+                      tuned_tree(L, m, nn, k, c), best, "tile=%dx%d bk=%d" % (
+                          TILES[c] + (TILE_K[c],)))
     return cfg, report
 
 
@@ -215,4 +228,4 @@ def tuned_tree(L, M, N, K, cfg):
     bm, bn = TILES[cfg]
     if M > bm:
         t = t.split(t.roots[0], bm)
-    return t.annotate(t.roots[0], "tile=%dx%d mfma_bf16" % (bm, bn))
+    return t.annotate(t.roots[0], "tile=%dx%d bk=%d mfma_bf16" % (bm, bn, TILE_K[cfg]))

@@ -11,8 +11,9 @@
 //            its shape bucket, ordered by decreasing cost so that the last
 //            tiles to start are the cheap ones; optionally captured once into
 //            a hipGraph and replayed.
-// The kernels are hand-written HIP (csrc/kernels/gemm.hip) built by hipcc
-// into kernels/gemm.hsaco next to this library.
+// The kernels are hand-written HIP (csrc/kernels/gemm.hip, on the tile of
+// csrc/kernels/igemm_tile.h) built by hipcc into kernels/gemm.hsaco next to
+// this library.
 #include <dlfcn.h>
 
 #include <algorithm>
@@ -40,8 +41,11 @@ struct Item {
   int gemm, m0, n0, cfg;
 };
 
-const int kTileM[5] = {64, 128, 64, 128, 256};
-const int kTileN[5] = {64, 64, 128, 128, 128};
+// tile configurations of csrc/kernels/gemm.hip (BM x BN x BK)
+constexpr int kNumCfg = 6;
+const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128};
+const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256};
+const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32};
 
 void chk(int rc, const char *what) {
   ASSERT(rc == 0) << what << ": " << lt_hip_last_error();
@@ -72,7 +76,7 @@ GemmModule &gemm_module() {
     loop_tool::hip::ensure_runtime();
     std::string path = library_dir() + "/kernels/gemm.hsaco";
     std::ifstream f(path, std::ios::binary);
-    ASSERT(f) << "kernel library missing: " << path << " (python -m loop_tool_amd.build)";
+    ASSERT(f.is_open()) << "kernel library missing: " << path << " (python -m loop_tool_amd.build)";
     auto *mm = new GemmModule();
     mm->code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
     chk(lt_hip_module_load(mm->code.data(), &mm->mod), "load gemm.hsaco");
@@ -145,7 +149,7 @@ struct lth_batch {
       int c = cfg_all >= 0 ? cfg_all : cfg[i];
       auto &d = descs[i];
       std::string name = std::string("lt_gemm_") + tname() + "_" + std::to_string(kTileM[c]) +
-                         "x" + std::to_string(kTileN[c]);
+                         "x" + std::to_string(kTileN[c]) + "x" + std::to_string(kTileK[c]);
       void *fn = gemm_module().fn(name);
       const void *A = d.A, *Bt = d.Bt;
       float *C = d.C;
@@ -215,7 +219,7 @@ int lth_batch_set_operands(lth_batch *b, void *const *A, void *const *Bt, void *
 int lth_batch_set_configs(lth_batch *b, const int *cfg) {
   return guarded([&] {
     for (size_t i = 0; i < b->descs.size(); ++i) {
-      ASSERT(cfg[i] >= 0 && cfg[i] < 5) << "tile config out of range";
+      ASSERT(cfg[i] >= 0 && cfg[i] < kNumCfg) << "tile config out of range";
       b->cfg[i] = cfg[i];
     }
     b->dirty = true;
@@ -232,7 +236,7 @@ int lth_batch_num_items(lth_batch *b) {
 
 int lth_batch_run_plain(lth_batch *b, int cfg, void *stream) {
   return guarded([&] {
-    ASSERT(cfg >= -1 && cfg < 5) << "tile config out of range";
+    ASSERT(cfg >= -1 && cfg < kNumCfg) << "tile config out of range";
     b->plain(cfg, stream);
     return 0;
   });
@@ -288,7 +292,7 @@ int lth_convert(const float *src, void *dst, int rows, int cols, int transpose, 
 }
 
 int lth_tile_config(int cfg, int *bm, int *bn) {
-  if (cfg < 0 || cfg >= 5) return -1;
+  if (cfg < 0 || cfg >= kNumCfg) return -1;
   *bm = kTileM[cfg];
   *bn = kTileN[cfg];
   return 0;

@@ -1,14 +1,18 @@
 // Kernel library for batches of run-time-sized matmul nests (gfx950).
 //
-//   lt_gemm_<t>_<BM>x<BN>(A, Bt, C, M, N, K)     one GEMM per launch
-//   lt_ggemm_<t>(descs, items, n_items)          persistent grouped launch
-//   lt_cvt_<t>(src, dst, rows, cols, transpose)  float32 -> 16-bit operands
+//   lt_gemm_<t>_<BM>x<BN>x<BK>(A, Bt, C, M, N, K)   one GEMM per launch
+//   lt_ggemm_<t>(descs, items, n_items)              persistent grouped launch
+//   lt_cvt_<t>(src, dst, rows, cols, transpose)      float32 -> 16-bit operands
 //
-// The grouped kernel walks a host-built work list of output tiles from many
-// GEMMs (ordered by decreasing cost); workgroup w handles items w, w+grid,
-// w+2*grid, ...  Every item carries its own tile configuration, chosen per
+// Both launch forms run the implicit-GEMM tile of igemm_tile.h on operands
+// A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
+// kernel walks a host-built work list of output tiles from many GEMMs
+// (ordered by decreasing cost); workgroup w handles items w, w + grid,
+// w + 2 grid, ...  Every item carries its own tile configuration, chosen per
 // shape bucket by the autotuner, so one launch mixes tile shapes.
-#include "gemm_tile.h"
+#include <hip/hip_runtime.h>
+
+#include "igemm_tile.h"
 
 struct LtGemmDesc {
   const void* A;
@@ -21,55 +25,83 @@ struct LtWorkItem {
   int gemm, m0, n0, cfg;
 };
 
-// tile configurations: index -> (BM, BN)
-#define LT_CFG_LIST(_) _(0, 64, 64) _(1, 128, 64) _(2, 64, 128) _(3, 128, 128) _(4, 256, 128)
+// geometry of a plain matmul for the tile: rows of A and Bt are K apart
+struct LtMmGeo {
+  int KO, KI;
+  long MD, ND, N;
+  __device__ __forceinline__ long p_row(long i, bool& ok) const {
+    ok = i < MD;
+    return i * KI;
+  }
+  __device__ __forceinline__ long q_row(long j, bool& ok) const {
+    ok = j < ND;
+    return j * KI;
+  }
+  __device__ __forceinline__ long p_koff(int) const { return 0; }
+  __device__ __forceinline__ long q_koff(int) const { return 0; }
+  __device__ __forceinline__ long out_row(long i) const { return i * N; }
+  __device__ __forceinline__ long out_col(long j) const { return j; }
+};
 
-constexpr int lt_max_lds() {
-  return lt_tile_lds_bytes<256, 128>();
+__device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
+  LtMmGeo g;
+  g.KO = 1;
+  g.KI = K;
+  g.MD = M;
+  g.ND = N;
+  g.N = N;
+  return g;
 }
 
-#define LT_PLAIN(T, TN, BM, BN)                                                            \
-  extern "C" __global__ void __launch_bounds__(256)                                        \
-      lt_gemm_##TN##_##BM##x##BN(const T* __restrict__ A, const T* __restrict__ Bt,        \
-                                 float* __restrict__ C, int M, int N, int K) {             \
-    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_tile_lds_bytes<BM, BN>()]; \
-    lt_gemm_tile<T, BM, BN>(A, Bt, C, M, N, K, blockIdx.y * BM, blockIdx.x * BN, smem);    \
+// tile configurations: index -> (BM, BN, BK); LDS of the grouped kernel is
+// the largest of them (128x128x64: 73,728 B, two workgroups per CU)
+#define LT_CFG_LIST(_) \
+  _(0, 64, 64, 32) _(1, 128, 64, 32) _(2, 64, 128, 32) _(3, 128, 128, 64) _(4, 256, 128, 32) \
+  _(5, 128, 256, 32)
+
+#define LT_PLAIN(T, TN, BM, BN, BK)                                                          \
+  extern "C" __global__ void __launch_bounds__(256)                                         \
+      lt_gemm_##TN##_##BM##x##BN##x##BK(const T* __restrict__ A, const T* __restrict__ Bt,  \
+                                        float* __restrict__ C, int M, int N, int K) {       \
+    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<BM, BN, BK>()]; \
+    lt_igemm_tile<T, BM, BN, BK>(lt_mm_geo(M, N, K), A, Bt, C, (long)blockIdx.y * BM,     \
+                                 (long)blockIdx.x * BN, smem);                             \
   }
 
-#define LT_CASE(T) \
-  switch (it.cfg)
+#define LT_CASE(i, BM, BN, BK)                                                               \
+  case i:                                                                                    \
+    lt_igemm_tile<T, BM, BN, BK>(g, A, Bt, d.C, it.m0, it.n0, smem);                        \
+    break;
 
-#define LT_GROUPED(T, TN)                                                                   \
+#define LT_GROUPED(T_, TN)                                                                   \
   extern "C" __global__ void __launch_bounds__(256)                                         \
       lt_ggemm_##TN(const LtGemmDesc* __restrict__ descs, const LtWorkItem* __restrict__ items, \
                     int n_items) {                                                          \
-    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_max_lds()];               \
+    typedef T_ T;                                                                           \
+    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<128, 128, 64>()]; \
     for (int i = blockIdx.x; i < n_items; i += gridDim.x) {                                 \
       const LtWorkItem it = items[i];                                                       \
       const LtGemmDesc d = descs[it.gemm];                                                  \
       const T* A = (const T*)d.A;                                                           \
       const T* Bt = (const T*)d.Bt;                                                         \
-      switch (it.cfg) {                                                                     \
-        case 0: lt_gemm_tile<T, 64, 64>(A, Bt, d.C, d.M, d.N, d.K, it.m0, it.n0, smem); break;   \
-        case 1: lt_gemm_tile<T, 128, 64>(A, Bt, d.C, d.M, d.N, d.K, it.m0, it.n0, smem); break;  \
-        case 2: lt_gemm_tile<T, 64, 128>(A, Bt, d.C, d.M, d.N, d.K, it.m0, it.n0, smem); break;  \
-        case 3: lt_gemm_tile<T, 128, 128>(A, Bt, d.C, d.M, d.N, d.K, it.m0, it.n0, smem); break; \
-        default: lt_gemm_tile<T, 256, 128>(A, Bt, d.C, d.M, d.N, d.K, it.m0, it.n0, smem); break; \
-      }                                                                                     \
+      const LtMmGeo g = lt_mm_geo(d.M, d.N, d.K);                                           \
+      switch (it.cfg) { LT_CFG_LIST(LT_CASE) default: break; }                              \
       __syncthreads(); /* LDS reuse by the next item */                                    \
     }                                                                                       \
   }
 
-#define LT_ALL(T, TN)        \
-  LT_PLAIN(T, TN, 64, 64)    \
-  LT_PLAIN(T, TN, 128, 64)   \
-  LT_PLAIN(T, TN, 64, 128)   \
-  LT_PLAIN(T, TN, 128, 128)  \
-  LT_PLAIN(T, TN, 256, 128)  \
-  LT_GROUPED(T, TN)
+#define LT_PLAIN_ALL(T, TN)                                                                \
+  LT_PLAIN(T, TN, 64, 64, 32)                                                              \
+  LT_PLAIN(T, TN, 128, 64, 32)                                                             \
+  LT_PLAIN(T, TN, 64, 128, 32)                                                             \
+  LT_PLAIN(T, TN, 128, 128, 64)                                                            \
+  LT_PLAIN(T, TN, 256, 128, 32)                                                            \
+  LT_PLAIN(T, TN, 128, 256, 32)
 
-LT_ALL(__bf16, bf16)
-LT_ALL(_Float16, f16)
+LT_PLAIN_ALL(__bf16, bf16)
+LT_PLAIN_ALL(_Float16, f16)
+LT_GROUPED(__bf16, bf16)
+LT_GROUPED(_Float16, f16)
 
 // float32 [rows][cols] -> 16-bit [rows][cols], or [cols][rows] when transpose
 #define LT_CVT(T, TN)                                                                       \

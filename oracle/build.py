@@ -16,6 +16,7 @@ __graft_entry__.smoke() and bench.py's cpu_baseline leg.
 """
 import concurrent.futures as cf
 import os
+import sys
 import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -109,8 +110,12 @@ def build_reference_plugin(force=False, jobs=8):
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     objdir = os.path.join(REF_OUT, "obj_pub")
     os.makedirs(objdir, exist_ok=True)
+    sys.path.insert(0, ROOT)
+    from loop_tool_amd import build as product_build
+    product_build._embed_sources()  # device headers the hip sources embed
     base = ["-std=c++17", "-O2", "-fPIC", "-I" + os.path.join(REF, "include"),
-            "-I" + os.path.join(ROOT, "include"), "-include", "chrono", "-w"]
+            "-I" + os.path.join(ROOT, "include"), "-include", "chrono", "-w",
+            "-I" + os.path.join(product_build.BUILD, "gen")]
     core = [os.path.join(REF, s) for s in REF_SOURCES] + [
         os.path.join(ROOT, "loop_tool_amd", "csrc", "capi.cpp")]
     hipdir = os.path.join(ROOT, "loop_tool_amd", "csrc", "hip")
