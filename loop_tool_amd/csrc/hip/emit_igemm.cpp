@@ -3,11 +3,11 @@
 // For a contraction nest (see emit_contract.cpp for the analysis) whose two
 // operands share a reduced var k_in that indexes one dim of each operand
 // alone (coefficient 1, no offset), the plan is
-//   stage 0, 1  one pass per operand: permute the operand so k_in's dim is
-//               innermost, pad every dim that a view can index out of range
-//               with a zero halo, zero-pad k_in to a multiple of the K
-//               tile, round to bf16/fp16 (RNE);
-//   stage 2     the implicit-GEMM tile of csrc/kernels/igemm_tile.h: every
+//   stage 0     one launch that stages both operands: permute each so
+//               k_in's dim is innermost, pad every dim that a view can index
+//               out of range with a zero halo, zero-pad k_in to a multiple
+//               of the K tile, round to bf16/fp16 (RNE);
+//   stage 1     the implicit-GEMM tile of csrc/kernels/igemm_tile.h: every
 //               operand access is now row_base(row) + outer_offset(k_out) +
 //               k_in with no bounds checks, loaded as 16-byte vectors.
 // For a 3x3 pad-1 convolution X[N][C][H][W] * W[Co][C][3][3] this stages
@@ -103,37 +103,12 @@ int source_contiguous_dim(const Operand &op, const Staged &s) {
   return ld;
 }
 
-// Staging as a transpose through LDS, for an operand whose source-contiguous
-// dim `ld` is not the staged inner dim (a convolution input X[N][C][H][W]
-// staged as X'[N][H+2][W+2][C]): a workgroup reads a 64 (ld) x TI (inner)
-// tile with lanes along ld (coalesced reads), and writes it back with
-// TI/8 lanes covering the 16-byte chunks of one ld position (coalesced
-// writes).
-std::string staging_transpose_kernel(const std::string &name, const Operand &op,
-                                     const Staged &s, int ld, int64_t kin, const char *elt,
-                                     unsigned grid[3]) {
-  const int nd = (int)op.sizes.size(), in = s.inner_dim;
-  const int TL = 64, TI = s.ssize[in] % 64 == 0 ? 64 : 32;
-  std::vector<int> other;
-  for (int q = 0; q < nd; ++q)
-    if (s.order[q] != ld && s.order[q] != in) other.push_back(s.order[q]);
-  int64_t nz = 1;
-  for (int d : other) nz *= s.ssize[d];
-  grid[0] = (unsigned)((s.ssize[ld] + TL - 1) / TL);
-  grid[1] = (unsigned)(s.ssize[in] / TI);
-  ASSERT(nz < 65536) << "staging grid too large";
-  grid[2] = (unsigned)nz;
-  auto idx = [&](int d, const std::string &sv) {
-    return "(" + sv + " - " + std::to_string(s.padlo[d]) + "L)";
-  };
-  std::stringstream k;
-  k << "extern \"C\" __global__ void __launch_bounds__(256) " << name
-    << "(const float* __restrict__ in, " << elt << "* __restrict__ S) {\n"
-    << "  typedef " << elt << " T8 __attribute__((ext_vector_type(8)));\n"
-    << "  __shared__ float tile[" << TI << "][" << TL + 1 << "];\n"
-    << "  const int tid = threadIdx.x;\n"
-    << "  const long l0 = (long)blockIdx.x * " << TL << ", c0 = (long)blockIdx.y * " << TI
-    << ";\n  long z_ = blockIdx.z;\n";
+// The fixed-dim prologue shared by the staging bodies: decomposes `z_` over
+// the staged dims in `other` (outer to inner) into s<d>, and builds the
+// validity test and source / staged offsets of those dims.
+void fixed_dims(std::stringstream &k, const Operand &op, const Staged &s,
+                const std::vector<int> &other, std::string &ok, std::string &src,
+                std::string &dst) {
   for (int q = (int)other.size() - 1; q >= 0; --q) {
     int d = other[q];
     if (q > 0)
@@ -142,16 +117,144 @@ std::string staging_transpose_kernel(const std::string &name, const Operand &op,
     else
       k << "  const long s" << d << " = z_;\n";
   }
-  // validity and offsets of the dims fixed for the workgroup
-  std::string ok = "true", base = "0L", dbase = "0L";
+  ok = "true";
+  src = "0L";
+  dst = "0L";
   for (int d : other) {
     const std::string sv = "s" + std::to_string(d);
+    const std::string idx = "(" + sv + " - " + std::to_string(s.padlo[d]) + "L)";
     if (s.ssize[d] != op.sizes[d])
-      ok += " && " + idx(d, sv) + " >= 0 && " + idx(d, sv) + " < " +
-            std::to_string(op.sizes[d]) + "L";
-    base += " + " + idx(d, sv) + " * " + std::to_string(op.strides[d]) + "L";
-    dbase += " + " + sv + " * " + std::to_string(s.sstride[d]) + "L";
+      ok += " && " + idx + " >= 0 && " + idx + " < " + std::to_string(op.sizes[d]) + "L";
+    src += " + " + idx + " * " + std::to_string(op.strides[d]) + "L";
+    dst += " + " + sv + " * " + std::to_string(s.sstride[d]) + "L";
   }
+}
+
+// Staging of whole source planes through LDS, for an operand whose two
+// fastest source dims are contiguous rows (ld: stride 1; a: stride =
+// sizes[ld]) and whose staged inner dim is a third dim (a convolution input
+// X[N][C][H][W] staged as X'[N][H+2][W+2][C]).  A workgroup owns RA staged
+// rows of `a` and TI values of the inner dim: per inner value it reads the
+// RA source rows as one contiguous run (16-byte loads, 8 lanes per 128-byte
+// line, 8 inner values per wave instruction), rounds to 16 bits into an LDS
+// tile [position][inner] (row pitch TI + 4 elements: the lanes of one store
+// instruction hit distinct banks), then writes every staged position of the
+// rows, halo included, as 16-byte chunks of TI/8 consecutive inner values.
+int64_t staging_plane_body(std::stringstream &k, const std::string &name, const Operand &op,
+                           const Staged &s, int ld, int a, int64_t kin, const char *elt) {
+  const int nd = (int)op.sizes.size(), in = s.inner_dim;
+  const int TI = s.ssize[in] % 64 == 0 ? 64 : 32, TP = TI + 4;
+  const int64_t L = op.sizes[ld];
+  const int64_t RA = std::min<int64_t>(s.ssize[a], std::max<int64_t>(1, 256 / L));
+  const int64_t NP = RA * L, NQ = (NP + 3) / 4, NQP = (NQ + 7) / 8 * 8, CP = TI / 2;
+  const int64_t nA = (s.ssize[a] + RA - 1) / RA, nC = s.ssize[in] / TI;
+  std::vector<int> other;
+  for (int q = 0; q < nd; ++q)
+    if (s.order[q] != ld && s.order[q] != in && s.order[q] != a) other.push_back(s.order[q]);
+  int64_t nz = 1;
+  for (int d : other) nz *= s.ssize[d];
+  bool vec = L % 4 == 0 && op.strides[in] % 4 == 0;
+  for (int d : other) vec = vec && op.strides[d] % 4 == 0;
+  const int64_t rd_items = NQP * CP, wr_items = RA * s.ssize[ld] * (TI / 8);
+  k << "__device__ __forceinline__ void " << name << "(const float* __restrict__ in, " << elt
+    << "* __restrict__ S, unsigned bid) {\n"
+    << "  typedef " << elt << " T2 __attribute__((ext_vector_type(2)));\n"
+    << "  typedef " << elt << " T4 __attribute__((ext_vector_type(4)));\n"
+    << "  typedef " << elt << " T8 __attribute__((ext_vector_type(8)));\n"
+    << "  typedef float F4 __attribute__((ext_vector_type(4)));\n"
+    << "  __shared__ __attribute__((aligned(16))) " << elt << " tile[" << NP * TP << "];\n"
+    << "  const int tid = threadIdx.x;\n"
+    << "  const long c0 = (long)(bid % " << nC << "u) * " << TI << ";\n"
+    << "  const long a0 = (long)((bid / " << nC << "u) % " << nA << "u) * " << RA << ";\n"
+    << "  long z_ = bid / " << nC * nA << "u;\n";
+  std::string ok, src, dst;
+  fixed_dims(k, op, s, other, ok, src, dst);
+  const std::string pa = std::to_string(s.padlo[a]);
+  k << "  const bool fixed_ok = " << ok << ";\n"
+    << "  const long src0 = " << src << " + (a0 - " << pa << "L) * " << L << "L;\n"
+    << "  const long dst0 = " << dst << ";\n"
+    // positions [pv0, pv1) of the tile hold source rows
+    << "  const long pv0 = (a0 < " << pa << "L ? " << pa << "L - a0 : 0L) * " << L << "L;\n"
+    << "  long pv1 = " << op.sizes[a] << "L + " << pa << "L - a0;\n"
+    << "  pv1 = (pv1 < " << RA << "L ? pv1 : " << RA << "L) * " << L << "L;\n"
+    << "  const bool vec = " << (vec ? "true" : "false")
+    << " && (((unsigned long)in & 15ul) == 0ul);\n"
+    << "  #pragma unroll\n  for (int it = 0; it < " << (rd_items + 255) / 256 << "; ++it) {\n"
+    << "    const int e = tid + it * 256;\n";
+  if (rd_items % 256) k << "    if (e >= " << rd_items << ") break;\n";
+  k << "    const int rest = e >> 6;\n"
+    << "    const int q = (rest % " << NQP / 8 << ") * 8 + (e & 7);\n"
+    << "    const int cp = (rest / " << NQP / 8 << ") * 8 + ((e >> 3) & 7);\n"
+    << "    if (q >= " << NQ << ") continue;\n"
+    << "    const long p = (long)q * 4, ch = c0 + 2 * cp;\n"
+    << "    const long o0 = src0 + ch * " << op.strides[in] << "L + p, o1 = o0 + "
+    << op.strides[in] << "L;\n"
+    << "    float x0[4], x1[4];\n"
+    << "    if (vec && fixed_ok && p >= pv0 && p + 4 <= pv1 && ch + 1 < " << kin << "L) {\n"
+    << "      const F4 u = *(const F4*)(in + o0), w = *(const F4*)(in + o1);\n"
+    << "      #pragma unroll\n      for (int i = 0; i < 4; ++i) { x0[i] = u[i]; x1[i] = w[i]; }\n"
+    << "    } else {\n"
+    << "      #pragma unroll\n      for (int i = 0; i < 4; ++i) {\n"
+    << "        const bool pk = fixed_ok && p + i >= pv0 && p + i < pv1;\n"
+    << "        x0[i] = pk && ch < " << kin << "L ? in[o0 + i] : 0.0f;\n"
+    << "        x1[i] = pk && ch + 1 < " << kin << "L ? in[o1 + i] : 0.0f;\n"
+    << "      }\n    }\n"
+    << "    #pragma unroll\n    for (int i = 0; i < 4; ++i) {\n"
+    << "      if (p + i >= " << NP << "L) break;\n"
+    << "      T2 t; t[0] = (" << elt << ")x0[i]; t[1] = (" << elt << ")x1[i];\n"
+    << "      *(T2*)(tile + (p + i) * " << TP << " + 2 * cp) = t;\n"
+    << "    }\n  }\n"
+    << "  __syncthreads();\n"
+    << "  #pragma unroll\n  for (int it = 0; it < " << (wr_items + 255) / 256 << "; ++it) {\n"
+    << "    const int e = tid + it * 256;\n";
+  if (wr_items % 256) k << "    if (e >= " << wr_items << ") break;\n";
+  const std::string pl = std::to_string(s.padlo[ld]);
+  k << "    const int g = e % " << TI / 8 << ";\n"
+    << "    const long sl = (e / " << TI / 8 << ") % " << s.ssize[ld] << ", r = e / "
+    << (TI / 8) * s.ssize[ld] << ";\n"
+    << "    if (a0 + r >= " << s.ssize[a] << "L) break;\n"
+    << "    const long w = sl - " << pl << "L, ra = a0 + r - " << pa << "L;\n"
+    << "    T8 v = {};\n"
+    << "    if (fixed_ok && w >= 0 && w < " << L << "L && ra >= 0 && ra < " << op.sizes[a]
+    << "L) {\n"
+    << "      const T4* t4 = (const T4*)(tile + (r * " << L << "L + w) * " << TP << " + g * 8);\n"
+    << "      const T4 lo = t4[0], hi = t4[1];\n"
+    << "      #pragma unroll\n      for (int j = 0; j < 4; ++j) { v[j] = lo[j]; v[j + 4] = hi[j]; }\n"
+    << "    }\n"
+    << "    *(T8*)(S + dst0 + (a0 + r) * " << s.sstride[a] << "L + sl * " << s.sstride[ld]
+    << "L + c0 + g * 8) = v;\n  }\n}\n";
+  return nz * nA * nC;
+}
+
+// Staging as a transpose through LDS, for an operand whose source-contiguous
+// dim `ld` is not the staged inner dim (B[K][N] of a matmul staged as
+// Bt[N][K]): a workgroup reads a 64 (ld) x TI (inner) tile with lanes along
+// ld (coalesced reads), and writes it back with TI/8 lanes covering the
+// 16-byte chunks of one ld position (coalesced writes).
+int64_t staging_transpose_body(std::stringstream &k, const std::string &name,
+                               const Operand &op, const Staged &s, int ld, int64_t kin,
+                               const char *elt) {
+  const int nd = (int)op.sizes.size(), in = s.inner_dim;
+  const int TL = 64, TI = s.ssize[in] % 64 == 0 ? 64 : 32;
+  std::vector<int> other;
+  for (int q = 0; q < nd; ++q)
+    if (s.order[q] != ld && s.order[q] != in) other.push_back(s.order[q]);
+  int64_t nz = 1;
+  for (int d : other) nz *= s.ssize[d];
+  const int64_t gx = (s.ssize[ld] + TL - 1) / TL, gy = s.ssize[in] / TI;
+  k << "__device__ __forceinline__ void " << name << "(const float* __restrict__ in, " << elt
+    << "* __restrict__ S, unsigned bid) {\n"
+    << "  typedef " << elt << " T8 __attribute__((ext_vector_type(8)));\n"
+    << "  __shared__ float tile[" << TI << "][" << TL + 1 << "];\n"
+    << "  const int tid = threadIdx.x;\n"
+    << "  const long l0 = (long)(bid % " << gx << "u) * " << TL << ", c0 = (long)((bid / "
+    << gx << "u) % " << gy << "u) * " << TI << ";\n"
+    << "  long z_ = bid / " << gx * gy << "u;\n";
+  std::string ok, base, dbase;
+  fixed_dims(k, op, s, other, ok, base, dbase);
+  auto idx = [&](int d, const std::string &sv) {
+    return "(" + sv + " - " + std::to_string(s.padlo[d]) + "L)";
+  };
   std::string lok;
   if (s.ssize[ld] != op.sizes[ld])
     lok = " && " + idx(ld, "sl") + " >= 0 && " + idx(ld, "sl") + " < " +
@@ -174,14 +277,14 @@ std::string staging_transpose_kernel(const std::string &name, const Operand &op,
     << "    T8 v;\n    #pragma unroll\n    for (int j = 0; j < 8; ++j) v[j] = (" << elt
     << ")tile[g * 8 + j][l];\n"
     << "    *(T8*)(S + dst0 + sl * " << s.sstride[ld] << "L + c0 + g * 8) = v;\n  }\n}\n";
-  return k.str();
+  return gx * gy * nz;
 }
 
 // One thread per 8 staged elements along the innermost staged dim.  Lanes
 // walk the dim that is contiguous in the source (when it is not the innermost
 // staged dim) so the source reads coalesce.
-std::string staging_kernel(const std::string &name, const Operand &op, const Staged &s,
-                           int64_t kin, const char *elt, int64_t *threads) {
+int64_t staging_generic_body(std::stringstream &k, const std::string &name, const Operand &op,
+                             const Staged &s, int64_t kin, const char *elt) {
   const int nd = (int)op.sizes.size();
   const int in = s.inner_dim;
   const int ld = source_contiguous_dim(op, s);
@@ -192,13 +295,11 @@ std::string staging_kernel(const std::string &name, const Operand &op, const Sta
   for (int q = nd - 2; q >= 0; --q)
     if (s.order[q] != ld) tord.push_back({s.order[q], s.ssize[s.order[q]]});
   const int64_t nvec = s.total / 8;
-  *threads = nvec;
 
-  std::stringstream k;
-  k << "extern \"C\" __global__ void __launch_bounds__(256) " << name
-    << "(const float* __restrict__ in, " << elt << "* __restrict__ S) {\n"
+  k << "__device__ __forceinline__ void " << name << "(const float* __restrict__ in, " << elt
+    << "* __restrict__ S, unsigned bid) {\n"
     << "  typedef " << elt << " T8 __attribute__((ext_vector_type(8)));\n"
-    << "  long t_ = (long)blockIdx.x * 256 + threadIdx.x;\n"
+    << "  long t_ = (long)bid * 256 + threadIdx.x;\n"
     << "  if (t_ >= " << nvec << "L) return;\n";
   for (int d = 0; d < nd; ++d) k << "  long s" << d << ";\n";
   for (size_t q = 0; q < tord.size(); ++q) {
@@ -224,7 +325,23 @@ std::string staging_kernel(const std::string &name, const Operand &op, const Sta
     << "    v[e] = ok && s" << in << " + e < " << kin << "L ? (" << elt << ")in[" << src
     << " + e * " << op.strides[in] << "L] : (" << elt << ")0.0f;\n"
     << "  *(T8*)(S + " << dst << ") = v;\n}\n";
-  return k.str();
+  return (nvec + 255) / 256;
+}
+
+// Emits the staging of one operand as `__device__ void name(in, S, bid)` and
+// returns its workgroup count; picks the plane, transpose or generic form.
+int64_t staging_body(std::stringstream &k, const std::string &name, const Operand &op,
+                     const Staged &s, int64_t kin, const char *elt) {
+  const int nd = (int)op.sizes.size(), in = s.inner_dim;
+  const int ld = source_contiguous_dim(op, s);
+  if (ld < 0 || op.strides[ld] != 1 || s.ssize[in] % 32 != 0)
+    return staging_generic_body(k, name, op, s, kin, elt);
+  for (int a = 0; a < nd; ++a)
+    if (a != ld && a != in && op.strides[a] == op.sizes[ld] && s.ssize[a] > 1 &&
+        op.sizes[ld] <= 512)
+      return staging_plane_body(k, name, op, s, ld, a, kin, elt);
+  if (s.ssize[ld] >= 32) return staging_transpose_body(k, name, op, s, ld, kin, elt);
+  return staging_generic_body(k, name, op, s, kin, elt);
 }
 
 std::string lin_over(const std::map<int, int64_t> &coef, const std::vector<VarInfo> &vs) {
@@ -317,26 +434,30 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
 
   std::stringstream k;
   k << kernel_prelude() << igemm_tile_source() << "\n";
-  for (int o = 0; o < 2; ++o) {
-    std::string name = "lt_stage" + std::to_string(o);
+  // ---- both staging passes in one launch: workgroups [0, nb[0]) stage
+  // operand 0, the rest operand 1 ----
+  int64_t nb[2];
+  for (int o = 0; o < 2; ++o)
+    nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt);
+  ASSERT(nb[0] + nb[1] < (1LL << 31)) << "staging grid too large";
+  k << "static constexpr unsigned lt_stage_blocks[2] = {" << nb[0] << "u, " << nb[1] << "u};\n"
+    << "extern \"C\" __global__ void __launch_bounds__(256) lt_stage0(const float* __restrict__ "
+       "in0, "
+    << elt << "* __restrict__ S0, const float* __restrict__ in1, " << elt
+    << "* __restrict__ S1) {\n"
+    << "  if (blockIdx.x < lt_stage_blocks[0]) lt_stage_op0(in0, S0, blockIdx.x);\n"
+    << "  else lt_stage_op1(in1, S1, blockIdx.x - lt_stage_blocks[0]);\n}\n";
+  {
     Stage s;
-    const int ld = source_contiguous_dim(c.ops[o], st[o]);
-    if (ld >= 0 && c.ops[o].strides[ld] == 1 && st[o].ssize[ld] >= 32 &&
-        st[o].ssize[st[o].inner_dim] % 32 == 0 &&
-        st[o].total / (st[o].ssize[ld] * st[o].ssize[st[o].inner_dim]) < 65536) {
-      k << staging_transpose_kernel(name, c.ops[o], st[o], ld, KI, elt, s.grid);
-    } else {
-      int64_t threads = 0;
-      k << staging_kernel(name, c.ops[o], st[o], KI, elt, &threads);
-      s.grid[0] = (unsigned)((threads + 255) / 256);
-    }
-    s.kernel = name;
+    s.kernel = "lt_stage0";
     s.kind = "stage";
-    s.buffers = {c.ops[o].input_index, sbuf[o]};
-    s.writes = {sbuf[o]};
+    s.buffers = {c.ops[0].input_index, sbuf[0], c.ops[1].input_index, sbuf[1]};
+    s.writes = {sbuf[0], sbuf[1]};
+    s.grid[0] = (unsigned)(nb[0] + nb[1]);
     s.block[0] = 256;
-    s.bytes = df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
-              st[o].total * 2;
+    for (int o = 0; o < 2; ++o)
+      s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
+                 st[o].total * 2;
     plan.stages.push_back(s);
   }
   // geometry of the tile

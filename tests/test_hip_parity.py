@@ -88,6 +88,8 @@ CONTRACT = [
     ("matmul", dict(M=300, N=200, K=704)),
     ("conv2d", dict(N=2, Ci=16, Co=72, H=15, W=15, stride=2)),
     ("conv2d", dict(N=4, Ci=64, Co=128, H=28, W=28)),
+    ("conv2d", dict(N=1, Ci=32, Co=8, H=9, W=12, pad=2, k=5)),
+    ("conv2d", dict(N=1, Ci=40, Co=24, H=5, W=300)),
 ]
 
 
@@ -146,8 +148,8 @@ STAGED = CONTRACT
 @pytest.mark.parametrize("tile", ["", "tile=64x64", "tile=128x64", "tile=64x128", "tile=128x256",
                                   "tile=128x256 bk=64", "tile=64x64 bk=64", "contract=gather"])
 def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
-    """Staged-operand lowering (operands permuted, zero-haloed and rounded to
-    bf16 by their own stages, then the implicit-GEMM tile) at every tile shape,
+    """Staged-operand lowering (both operands permuted, zero-haloed and rounded
+    to bf16 by one staging launch, then the implicit-GEMM tile) at every tile shape,
     and the gathered lowering, against float64 of the bf16-rounded operands."""
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
@@ -157,7 +159,7 @@ def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
     cc = lt.compile(t, "hip_bf16")
     kinds = cc.string_property("kinds")
     assert kinds == ("contract_bf16" if tile == "contract=gather" else
-                     "stage,stage,contract_bf16"), kinds
+                     "stage,contract_bf16"), kinds
     y = np.zeros(t.io_numel()[len(inputs)], np.float32)
     cc.run(inputs + [y])
 

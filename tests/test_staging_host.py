@@ -1,0 +1,188 @@
+"""Staging passes of the 16-bit contraction lowering, executed on the CPU.
+
+The hip backend emits the staging of each contraction operand as a device
+function `lt_stage_op<o>(in, S, workgroup)`, called by one fused staging
+kernel.  Here those functions are compiled for the host with ROCm's clang and
+run workgroup by workgroup: all 256 threads run up to the barrier, then all
+run again through the write phase (the LDS tile is a static array, and the
+pre-barrier phase is idempotent).  The staged buffers must hold exactly the
+layout the MFMA tile reads: the source permuted so the shared reduced dim is
+innermost, a zero halo wherever a view indexes out of range, zeros up to the
+padded K extent, values rounded to bf16 (RNE).  The host build runs under
+AddressSanitizer, so a read outside an input fails; guard words after each
+staged buffer catch writes past its end, and a nonzero fill catches elements
+never written.
+Expected layouts come from numpy relayouts of the inputs, not from the
+emitter.
+"""
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from loop_tool_amd import programs as P
+
+CLANG = "/opt/rocm/llvm/bin/clang++"
+
+PRELUDE = r"""
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+struct { unsigned x; } threadIdx;
+static int g_phase;
+#define __device__
+#define __forceinline__ inline
+#define __shared__ static
+#define __syncthreads() do { if (g_phase == 0) return; } while (0)
+"""
+
+DRIVER = r"""
+typedef void (*stage_fn)(const float*, __bf16*, unsigned);
+static void* slurp(const char* path, size_t n) {
+  void* p = malloc(n);
+  FILE* fp = fopen(path, "rb");
+  if (!fp || fread(p, 1, n, fp) != n) { fprintf(stderr, "cannot read %s\n", path); exit(2); }
+  fclose(fp);
+  return p;
+}
+int main(int argc, char** argv) {
+  // argv: dir numel0 staged0 numel1 staged1
+  stage_fn fn[2] = {lt_stage_op0, lt_stage_op1};
+  for (int o = 0; o < 2; ++o) {
+    size_t n = strtoull(argv[2 + 2 * o], 0, 10), s = strtoull(argv[3 + 2 * o], 0, 10);
+    char path[4096];
+    snprintf(path, sizeof path, "%s/in%d.bin", argv[1], o);
+    float* in = (float*)slurp(path, n * 4);
+    unsigned short* S = (unsigned short*)malloc((s + 64) * 2);
+    memset(S, 0xA5, (s + 64) * 2);
+    for (unsigned b = 0; b < lt_stage_blocks[o]; ++b)
+      for (g_phase = 0; g_phase < 2; ++g_phase)
+        for (threadIdx.x = 0; threadIdx.x < 256; ++threadIdx.x) fn[o](in, (__bf16*)S, b);
+    snprintf(path, sizeof path, "%s/S%d.bin", argv[1], o);
+    FILE* fp = fopen(path, "wb");
+    fwrite(S, 2, s + 64, fp);
+    fclose(fp);
+    free(in);
+    free(S);
+  }
+  return 0;
+}
+"""
+
+
+def bf16_bits(a):
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def extract(src, name):
+    """The text of one emitted device function, by brace matching."""
+    i = src.index("__device__ __forceinline__ void " + name + "(")
+    depth, j = 0, src.index("{", i)
+    while True:
+        depth += {"{": 1, "}": -1}.get(src[j], 0)
+        j += 1
+        if depth == 0:
+            return src[i:j]
+
+
+def staged_conv_input(X, pad, k, stride, kip):
+    """X[N][C][H][W] -> X'[N][H+lo+hi][W+lo+hi][kip] (bf16 bits)."""
+    N, C, H, W = X.shape
+    Ho, Wo = (H + 2 * pad - k) // stride + 1, (W + 2 * pad - k) // stride + 1
+    hi_h = max(0, (Ho - 1) * stride + k - 1 - pad - (H - 1))
+    hi_w = max(0, (Wo - 1) * stride + k - 1 - pad - (W - 1))
+    out = np.zeros((N, H + pad + hi_h, W + pad + hi_w, kip), np.uint16)
+    out[:, pad:pad + H, pad:pad + W, :C] = bf16_bits(X.transpose(0, 2, 3, 1))
+    return out
+
+
+def staged_weights(Wt, kip):
+    """W[Co][C][k][k] -> W'[Co][k][k][kip] (bf16 bits)."""
+    Co, C, k, _ = Wt.shape
+    out = np.zeros((Co, k, k, kip), np.uint16)
+    out[..., :C] = bf16_bits(Wt.transpose(0, 2, 3, 1))
+    return out
+
+
+def run_staging(lt, out, inputs, tmp):
+    cc = lt.compile(out.loop_tree, "hip_bf16")
+    assert cc.string_property("kinds") == "stage,contract_bf16"
+    src = cc.string_property("code")
+    blocks = re.search(r"static constexpr unsigned lt_stage_blocks\[2\] = \{[^}]*\};", src).group(0)
+    body = (PRELUDE + blocks + "\n" + extract(src, "lt_stage_op0") + "\n" +
+            extract(src, "lt_stage_op1") + "\n" + DRIVER)
+    with open(os.path.join(tmp, "s.cpp"), "w") as f:
+        f.write(body)
+    # AddressSanitizer (host build): a source read outside the input is an error
+    subprocess.run([CLANG, "-std=c++17", "-O1", "-w", "-fsanitize=address",
+                    "-o", os.path.join(tmp, "s"), os.path.join(tmp, "s.cpp")], check=True)
+    return cc
+
+
+def check_staged(tmp, exp, numels):
+    args = [os.path.join(tmp, "s"), tmp]
+    for o in range(2):
+        args += [str(numels[o]), str(exp[o].size)]
+    subprocess.run(args, check=True, timeout=300)
+    for o in range(2):
+        S = np.fromfile(os.path.join(tmp, "S%d.bin" % o), np.uint16)
+        assert np.all(S[exp[o].size:] == 0xA5A5), "operand %d: write past the staged buffer" % o
+        bad = np.flatnonzero(S[:exp[o].size] != exp[o].ravel())
+        assert bad.size == 0, "operand %d: %d wrong elements, first at %s of %s" % (
+            o, bad.size, np.unravel_index(bad[0], exp[o].shape), exp[o].shape)
+
+
+CONV = [
+    dict(N=2, Ci=128, Co=16, H=56, W=56),            # 16-byte loads, 64 channels per workgroup
+    dict(N=1, Ci=40, Co=8, H=15, W=15),              # scalar loads (odd rows), K 40 padded to 64
+    dict(N=2, Ci=16, Co=72, H=15, W=15, stride=2),   # stride-2 halo, 32 channels per workgroup
+    dict(N=1, Ci=32, Co=8, H=9, W=12, pad=2, k=5),   # 5x5 window, two-wide halo
+    dict(N=1, Ci=64, Co=8, H=5, W=300),              # rows longer than 256: one row per workgroup
+]
+
+
+@pytest.fixture
+def tmpdir_():
+    d = tempfile.mkdtemp(prefix="lt_stage_host_")
+    yield d
+    shutil.rmtree(d)
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("kw", CONV, ids=[str(i) for i in range(len(CONV))])
+def test_conv_staging_on_host(lt, kw, tmpdir_):
+    pad, k, stride = kw.get("pad", 1), kw.get("k", 3), kw.get("stride", 1)
+    out, shapes = P.conv2d(lt, **kw)
+    X, Wt = P.uniform(7, shapes[0]), P.uniform(8, shapes[1])
+    cc = run_staging(lt, out, [X, Wt], tmpdir_)
+    d = cc.dump()
+    kin = kw["Ci"]
+    m = re.search(r"K=\d+x%d(?: \(padded to (\d+)\))?" % kin, d)
+    kip = int(m.group(1)) if m.group(1) else kin
+    exp = [staged_conv_input(X, pad, k, stride, kip), staged_weights(Wt, kip)]
+    X.tofile(os.path.join(tmpdir_, "in0.bin"))
+    Wt.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, exp, [X.size, Wt.size])
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("M,N,K", [(64, 96, 128), (37, 53, 29), (300, 200, 704)])
+def test_matmul_staging_on_host(lt, M, N, K, tmpdir_):
+    """A[M][K] rounded in place (K padded), B[K][N] transposed to Bt[N][K]."""
+    out, shapes = P.matmul(lt, M, N, K)
+    A, B = P.uniform(9, shapes[0]), P.uniform(10, shapes[1])
+    cc = run_staging(lt, out, [A, B], tmpdir_)
+    m = re.search(r"K=1x%d(?: \(padded to (\d+)\))?" % K, cc.dump())
+    kip = int(m.group(1)) if m.group(1) else K
+    ea = np.zeros((M, kip), np.uint16)
+    ea[:, :K] = bf16_bits(A)
+    eb = np.zeros((N, kip), np.uint16)
+    eb[:, :K] = bf16_bits(B.T)
+    A.tofile(os.path.join(tmpdir_, "in0.bin"))
+    B.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, [ea, eb], [A.size, B.size])
