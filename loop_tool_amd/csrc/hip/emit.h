@@ -48,6 +48,8 @@ struct Knobs {
   int bk = 0;                  // "bk=N": K tile of 16-bit contractions (32, 64)
   bool gather = false;         // "contract=gather": 16-bit contractions without
                                // operand staging
+  bool igemm_plain = false;    // "igemm=plain": the single-stage tile instead of
+                               // the software-pipelined one
 };
 
 // Builds the whole launch plan for a LoopTree.

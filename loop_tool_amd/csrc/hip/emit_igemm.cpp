@@ -419,6 +419,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   const int lds = 2 * (BM + BN) * (BK + 8) * 2;
   const bool full = c.MD % BM == 0 && c.ND % BN == 0;
   const bool int32 = SP.total < (1LL << 31) && SQ.total < (1LL << 31);
+  // the pipelined tile addresses each operand with 32-bit byte offsets
+  const bool pipe = !knobs.igemm_plain && SP.total < (1LL << 30) && SQ.total < (1LL << 30);
 
   // ---- buffers: one staged (16-bit) scratch buffer per operand ----
   int sbuf[2];
@@ -492,10 +494,17 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << "  g.zp = " << lin_over(SP.scoef, c.B) << ";\n"
     << "  g.zq = " << lin_over(SQ.scoef, c.B) << ";\n"
     << "  g.zo = " << lin_over(c.ostride, c.B) << ";\n"
-    << "  lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
-    << (full ? "true" : "false") << ", " << (int32 ? "int" : "long")
-    << ">(g, P, Q, out, (long)blockIdx.y * " << BM
-    << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem);\n}\n";
+    ;
+  if (pipe)
+    k << "  lt_igemm_tile_pipe<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ">(g, P, Q, out, (long)blockIdx.y * " << BM
+      << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem, "
+      << SP.total * 2 << "u, " << SQ.total * 2 << "u);\n}\n";
+  else
+    k << "  lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << (int32 ? "int" : "long")
+      << ">(g, P, Q, out, (long)blockIdx.y * " << BM
+      << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem);\n}\n";
 
   Stage g;
   g.kernel = "lt_igemm0";
@@ -516,7 +525,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : "f16")
        << " staged operands, f32 accumulate): D[" << c.BD << "][" << c.MD << "][" << c.ND
        << "] = P x Q over K=" << ko_n << "x" << KI << (KIP != KI ? " (padded to " + std::to_string(KIP) + ")" : std::string()) << ", tile " << BM << "x" << BN << "x"
-       << BK << "; staged P " << SP.total << " and Q " << SQ.total << " elements\n";
+       << BK << (pipe ? " pipelined" : "") << "; staged P " << SP.total << " and Q " << SQ.total
+       << " elements\n";
   plan.description = desc.str();
   return true;
 }

@@ -23,6 +23,8 @@
 //   bk=N           K tile (32 or 64) of 16-bit contractions
 //   contract=gather  16-bit contractions gather their operands in the tile
 //                  loader instead of staging them
+//   igemm=plain    staged 16-bit contractions use the single-stage tile
+//                  (default: the software-pipelined tile where it applies)
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -166,6 +168,7 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       else if (tok == "mfma_bf16") k.bf16 = true;
       else if (tok == "mfma_f16") k.f16 = true;
       else if (tok == "contract=gather") k.gather = true;
+      else if (tok == "igemm=plain") k.igemm_plain = true;
       else if (tok.rfind("bk=", 0) == 0) k.bk = std::stoi(tok.substr(3));
       else if (tok.rfind("block=", 0) == 0) k.block = std::stoi(tok.substr(6));
       else if (tok.rfind("vec=", 0) == 0) k.vec = std::stoi(tok.substr(4));

@@ -7,7 +7,8 @@
 // i.e. the innermost reduction var (extent KI) is the contiguous dim and
 // every other var enters through an affine row or outer-K offset.  Staging
 // pads the inner dim with zeros to a multiple of BK, so KI % BK == 0 and no
-// K bounds checks remain (the check below is kept for callers that do not).  A matmul operand A[M][K] is already staged (KO = 1); a convolution
+// K bounds checks remain (lt_igemm_tile keeps the check for callers that do
+// not).  A matmul operand A[M][K] is already staged (KO = 1); a convolution
 // input X[N][C][H][W] staged as the zero-haloed X'[N][H+2][W+2][C] gives
 // row_base(n, ho, wo) + outer_offset(kh, kw) + ci with no bounds checks.
 //
@@ -21,7 +22,8 @@
 // A workgroup of 256 lanes (4 waves as 2 x 2) computes a BM x BN tile; each
 // wave owns (BM/2) x (BN/2) as 32x32 blocks of v_mfma_f32_32x32x16_{bf16,f16}.
 // K advances BK (32 or 64) at a time through double-buffered LDS: one barrier
-// per K tile, next tile's 16-byte global loads in flight during the MFMAs.
+// per K tile, next tile's 16-byte global loads in flight during the MFMAs
+// (lt_igemm_tile_pipe: two tiles ahead, with a pinned instruction interleave).
 // Rows of LDS are padded by 16 bytes so the 16-byte MFMA operand reads of 32
 // consecutive rows spread over the banks.
 //
@@ -36,6 +38,7 @@ typedef __bf16 lt_ig_bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 lt_ig_f16x8 __attribute__((ext_vector_type(8)));
 typedef float lt_ig_f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned short lt_ig_u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int lt_ig_u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename T> struct lt_ig_vec8;
 template <> struct lt_ig_vec8<__bf16> { typedef lt_ig_bf16x8 type; };
@@ -60,6 +63,29 @@ __device__ __forceinline__ unsigned lt_igemm_swizzle(unsigned b, unsigned n) {
 template <int BM, int BN, int BK>
 __host__ __device__ constexpr int lt_igemm_lds_bytes() {
   return 2 * (BM + BN) * (BK + 8) * 2;
+}
+
+// Writes a wave's TM x TN accumulator blocks (rows from i0, columns from j0).
+// 32x32 accumulator layout: col = lane % 32, row = (r/4)*8 + (lane/32)*4 + r%4.
+template <int TM, int TN, bool FULL, class G>
+__device__ __forceinline__ void lt_igemm_store(const G& g, const lt_ig_f32x16 (&acc)[TM][TN],
+                                               float* __restrict__ out, long i0, long j0,
+                                               int lane) {
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    const long j = j0 + b * 32 + (lane & 31);
+    if (!FULL && j >= g.ND) continue;
+    const long oj = g.out_col(j);
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long i = i0 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (FULL || i < g.MD)
+          __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
+      }
+    }
+  }
 }
 
 // Row offsets are computed once per tile in IT (int when every staged
@@ -157,22 +183,155 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
     }
     __syncthreads();
   }
-  // 32x32 accumulator layout: col = lane % 32, row = (r/4)*8 + (lane/32)*4 + r%4
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+}
+
+// Software-pipelined form of the same tile, for staged operands of fewer than
+// 2^31 bytes with KI % BK == 0 (pbytes, qbytes: operand sizes in bytes).
+//  * Global loads are buffer loads: a 32-bit byte offset per row in a VGPR
+//    and the K tile's offset in an SGPR; rows outside the problem get an
+//    offset beyond the buffer, which the hardware reads as 0.
+//  * Loads run two K tiles ahead into two register sets, so a tile's loads
+//    have a whole K tile of MFMAs to land before they are stored to LDS.
+//  * Within a K tile, the next k16 step's fragments are read during the
+//    current step's MFMAs, and the next tile's LDS stores are interleaved
+//    with the last step's MFMAs; one barrier per K tile.
+//  * sched_group_barrier pins the interleave in the steady state: one MFMA,
+//    then one fragment read and one global load (or one LDS store).
+// Each lane's working set: TM*TN*16 accumulators, 2*(TM+TN) fragments of 4
+// VGPRs and 2*(VP+VQ) load registers of 4.
+template <typename T, int BM, int BN, int BK, bool FULL, class G>
+__device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restrict__ P,
+                                                   const T* __restrict__ Q,
+                                                   float* __restrict__ out, long i0, long j0,
+                                                   unsigned char* smem, unsigned pbytes,
+                                                   unsigned qbytes) {
+  typedef typename lt_ig_vec8<T>::type V;
+  constexpr int NT = 256, LDR = BK + 8;
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 32, TN = WN / 32;
+  constexpr int CPR = BK / 8, KK = BK / 16;
+  constexpr int VP = BM * CPR / NT, VQ = BN * CPR / NT;
+  static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
+                "tile too small for 256 lanes");
+  static_assert(KK % 2 == 0, "BK must be a multiple of 32");
+  const int KIT = g.KI / BK;
+  const int KT = g.KO * KIT;
+  T* sP = (T*)smem;              // [2][BM][LDR]
+  T* sQ = sP + 2 * BM * LDR;     // [2][BN][LDR]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int cc = (tid % CPR) * 8;
+  const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
+  const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
+  unsigned pv[VP], qv[VQ];
 #pragma unroll
-  for (int b = 0; b < TN; ++b) {
-    const long j = j0 + wn * WN + b * 32 + (lane & 31);
-    if (!FULL && j >= g.ND) continue;
-    const long oj = g.out_col(j);
+  for (int v = 0; v < VP; ++v) {
+    bool ok;
+    const long o = g.p_row(i0 + (tid + v * NT) / CPR, ok);
+    pv[v] = (FULL || ok) ? (unsigned)((o + cc) * 2) : 0x80000000u;
+  }
 #pragma unroll
-    for (int a = 0; a < TM; ++a) {
+  for (int v = 0; v < VQ; ++v) {
+    bool ok;
+    const long o = g.q_row(j0 + (tid + v * NT) / CPR, ok);
+    qv[v] = (FULL || ok) ? (unsigned)((o + cc) * 2) : 0x80000000u;
+  }
+  lt_ig_u32x4 r0p[VP], r0q[VQ], r1p[VP], r1q[VQ];
+  auto load = [&](int kt, lt_ig_u32x4* rp, lt_ig_u32x4* rq) {
+    const int ko = kt / KIT, kb = (kt % KIT) * BK;
+    const int po = (int)((g.p_koff(ko) + kb) * 2), qo = (int)((g.q_koff(ko) + kb) * 2);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const long i = i0 + wm * WM + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (FULL || i < g.MD)
-          __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
+    for (int v = 0; v < VP; ++v)
+      rp[v] = __builtin_amdgcn_raw_buffer_load_b128(rsP, (int)pv[v], po, 0);
+#pragma unroll
+    for (int v = 0; v < VQ; ++v)
+      rq[v] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)qv[v], qo, 0);
+  };
+  auto store = [&](int buf, const lt_ig_u32x4* rp, const lt_ig_u32x4* rq) {
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      *(lt_ig_u32x4*)(sP + (buf * BM + (tid + v * NT) / CPR) * LDR + cc) = rp[v];
+#pragma unroll
+    for (int v = 0; v < VQ; ++v)
+      *(lt_ig_u32x4*)(sQ + (buf * BN + (tid + v * NT) / CPR) * LDR + cc) = rq[v];
+  };
+  V fa[2][TM], fb[2][TN];
+  auto frags = [&](int buf, int kk, int s) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[s][a] = *(const V*)(sP + (buf * BM + wm * WM + a * 32 + (lane & 31)) * LDR +
+                             kk * 16 + (lane >> 5) * 8);
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+      fb[s][b] = *(const V*)(sQ + (buf * BN + wn * WN + b * 32 + (lane & 31)) * LDR +
+                             kk * 16 + (lane >> 5) * 8);
+  };
+  lt_ig_f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  // One K tile: tile kt is in LDS buffer B and its k16-step-0 fragments in
+  // fa/fb[0]; RN holds tile kt+1 (loaded a tile ago); RF receives tile kt+2
+  // when ld.  With st, tile kt+1 goes to buffer B^1 and its first fragments
+  // are read after the barrier.
+  auto step = [&](int kt, int B, const lt_ig_u32x4* RNp, const lt_ig_u32x4* RNq,
+                  lt_ig_u32x4* RFp, lt_ig_u32x4* RFq, bool ld, bool st) {
+    if (ld) load(kt + 2, RFp, RFq);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const int s = kk & 1;
+      if (kk + 1 < KK) frags(B, kk + 1, s ^ 1);
+      if (kk == KK - 1 && st) store(B ^ 1, RNp, RNq);
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
+    }
+    if (ld && st) {
+      constexpr int NM = TM * TN, NR = TM + TN, NL = VP + VQ;
+#pragma unroll
+      for (int kk = 0; kk < KK - 1; ++kk)
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+          __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);                 // MFMA
+          if (i < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);    // LDS read
+          if (kk == 0 && i < NL) __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);  // load
+        }
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);                   // MFMA
+        if (i < NL) __builtin_amdgcn_sched_group_barrier(0x0200, 1, 0);      // LDS write
       }
     }
+    if (st) {
+      __syncthreads();
+      frags(B ^ 1, 0, 0);
+    }
+  };
+  load(0, r0p, r0q);
+  if (KT > 1) load(1, r1p, r1q);
+  store(0, r0p, r0q);
+  __syncthreads();
+  frags(0, 0, 0);
+  // tile t lives in register set t % 2 and LDS buffer t % 2
+  int kt = 0;
+  for (; kt + 3 < KT; kt += 2) {
+    step(kt, 0, r1p, r1q, r0p, r0q, true, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, true, true);
   }
+  const int R = KT - kt;  // 1, 2 or 3 tiles left; kt is even
+  if (R == 3) {
+    step(kt, 0, r1p, r1q, r0p, r0q, true, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, false, true);
+    step(kt + 2, 0, r1p, r1q, r0p, r0q, false, false);
+  } else if (R == 2) {
+    step(kt, 0, r1p, r1q, r0p, r0q, false, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, false, false);
+  } else {
+    step(kt, 0, r1p, r1q, r0p, r0q, false, false);
+  }
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
 }
 
 #endif  // LT_IGEMM_TILE_H_

@@ -146,11 +146,12 @@ STAGED = CONTRACT
 
 @pytest.mark.parametrize("name,kw", STAGED, ids=["%s%d" % (c[0], i) for i, c in enumerate(STAGED)])
 @pytest.mark.parametrize("tile", ["", "tile=64x64", "tile=128x64", "tile=64x128", "tile=128x256",
-                                  "tile=128x256 bk=64", "tile=64x64 bk=64", "contract=gather"])
+                                  "tile=128x256 bk=64", "tile=64x64 bk=64", "igemm=plain",
+                                  "tile=128x256 bk=64 igemm=plain", "contract=gather"])
 def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
     """Staged-operand lowering (both operands permuted, zero-haloed and rounded
-    to bf16 by one staging launch, then the implicit-GEMM tile) at every tile shape,
-    and the gathered lowering, against float64 of the bf16-rounded operands."""
+    to bf16 by one staging launch, then the implicit-GEMM tile, pipelined or
+    single-stage) at every tile shape, and the gathered lowering, against float64 of the bf16-rounded operands."""
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     t = out.loop_tree
