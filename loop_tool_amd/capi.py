@@ -6,7 +6,9 @@ Mirrors the Python API of the reference's pybind11 module `loop_tool_py`
 compiling a LoopTree with a named backend.  Any library exporting the C ABI can
 be wrapped (the product library, or another build of the same API).
 """
+import contextlib
 import ctypes as C
+import fcntl
 import os
 
 import numpy as np
@@ -28,8 +30,14 @@ class LoopToolError(RuntimeError):
 class Library:
     """One loaded implementation of the loop_tool C ABI."""
 
-    def __init__(self, path, mode=C.RTLD_LOCAL):
+    def __init__(self, path, mode=C.RTLD_LOCAL, compile_lock=None):
+        """compile_lock: path of a lock file held across every call that may
+        compile.  The reference's C backend writes its code to fixed names
+        (/tmp/fn_<n>.c and .so, src/backends/cpu/cpp.cpp:462-472), so two
+        processes compiling through it at once can load each other's half
+        written library; the lock serialises them."""
         self.path = path
+        self.compile_lock_path = compile_lock
         self.lib = C.CDLL(path, mode=mode)
         L = self.lib
         s = lambda *a: _sig(L, *a)
@@ -116,6 +124,18 @@ class Library:
         s("ltc_compiled_free", None, _P)
 
     # -- helpers ---------------------------------------------------------
+    @contextlib.contextmanager
+    def compiling(self):
+        if self.compile_lock_path is None:
+            yield
+            return
+        with open(self.compile_lock_path, "a") as f:
+            fcntl.flock(f, fcntl.LOCK_EX)
+            try:
+                yield
+            finally:
+                fcntl.flock(f, fcntl.LOCK_UN)
+
     def error(self):
         return self.lib.ltc_last_error().decode()
 
@@ -167,8 +187,9 @@ class Library:
         return LoopTree(self, self.check_ptr(self.lib.ltc_looptree_deserialize(text.encode())))
 
     def compile(self, loop_tree, backend):
-        return Compiled(self, self.check_ptr(
-            self.lib.ltc_compile(loop_tree.h, backend.encode())))
+        with self.compiling():
+            return Compiled(self, self.check_ptr(
+                self.lib.ltc_compile(loop_tree.h, backend.encode())))
 
     def convolve(self, X, W, spatial, window, stride=1):
         return Tensor(self, self.check_ptr(self.lib.ltc_nn_convolve(
@@ -423,19 +444,22 @@ class Tensor(_Handle):
 
     def numpy(self):
         sizes = self.sizes
-        p = self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
+        with self.L.compiling():
+            p = self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
         n = int(np.prod(sizes)) if sizes else 1
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(sizes).copy()
 
     def resolve(self):
-        self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
+        with self.L.compiling():
+            self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
 
     @property
     def code(self):
         return self.L.string(self.L.lib.ltc_tensor_code, self.h)
 
     def compile(self):
-        self.L.check_int(self.L.lib.ltc_tensor_compile(self.h))
+        with self.L.compiling():
+            self.L.check_int(self.L.lib.ltc_tensor_compile(self.h))
 
     @property
     def loop_tree(self):
@@ -446,7 +470,8 @@ class Tensor(_Handle):
 
     @property
     def compiled(self):
-        return Compiled(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_compiled(self.h)))
+        with self.L.compiling():
+            return Compiled(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_compiled(self.h)))
 
     def __hash__(self):
         return self.L.lib.ltc_tensor_hash(self.h)

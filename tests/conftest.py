@@ -34,7 +34,9 @@ def ref():
     if not os.path.exists(REF_LIB):
         pytest.skip("reference library oracle/_ref not built")
     from loop_tool_amd import capi
-    return capi.Library(REF_LIB)
+    # the reference's C backend compiles to fixed /tmp names: one compile at a
+    # time across the test processes
+    return capi.Library(REF_LIB, compile_lock="/tmp/lt_ref_cpp_backend.lock")
 
 
 @pytest.fixture(scope="session")
