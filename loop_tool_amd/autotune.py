@@ -3,7 +3,8 @@
 The search space is the LoopTree itself: candidate schedules are produced with
 the reference's schedule transforms (split, swap, merge, annotate;
 src/frontends/mutate.cpp) plus the hip backend's annotations (block=N, vec=N,
-tile=MxN, hip:plain, mfma_bf16).  Every candidate is compiled, checked against
+tile=MxN, bk=N, waves=N, stage=..., igemm=..., hip:plain, mfma_bf16).  Every
+candidate is compiled, checked against
 a baseline result on the same seeded inputs (the harness of the reference's
 test/test.py:85-123: a schedule must not change the result beyond the stated
 tolerance), and timed with HIP events on device-resident buffers.  The winner
@@ -62,8 +63,13 @@ def knob_candidates(kind):
     if kind == "rowreduce":
         return ["block=%d vec=%d" % (b, v) for b, v in
                 itertools.product((64, 128, 256, 512, 1024), (4, 2, 1))]
-    if kind.startswith("contract"):
+    if kind == "contract_f32":
         return ["tile=%dx%d" % t for t in ((64, 64), (128, 64), (64, 128), (128, 128))]
+    if kind.startswith("contract"):  # staged 16-bit operands
+        tiles = ["tile=128x256", "tile=256x128", "tile=128x128 bk=64", "tile=256x256 bk=64",
+                 "tile=64x128", "tile=128x64"]
+        return tiles + ["stage=flat", "waves=8", "waves=2", "igemm=halo",
+                        "igemm=halo tile=128x128", "igemm=plain"]
     if kind == "pointwise":
         return ["block=%d vec=%d" % (b, v) for b, v in
                 itertools.product((128, 256, 512), (4, 2, 1))]

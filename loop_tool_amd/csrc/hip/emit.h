@@ -50,6 +50,15 @@ struct Knobs {
                                // operand staging
   bool igemm_plain = false;    // "igemm=plain": the single-stage tile instead of
                                // the software-pipelined one
+  int waves = 0;               // "waves=2|4|8": waves per workgroup of the
+                               // pipelined 16-bit tile (0 = auto)
+  int lds_swz = -1;            // "lds=swz|pad": XOR-permuted or padded LDS rows
+                               // of the pipelined tile (-1 = auto)
+  int stage_blocked = -1;      // "stage=blocked|flat": k-blocked operand staging
+                               // (-1 = auto: blocked when k_in spans several K tiles)
+  int kb_outer = -1;           // "kb=outer|inner": order of the K blocks
+  int igemm_halo = -1;         // "igemm=halo": staged 16-bit contractions load
+                               // the Q rows of all taps of a K block once
 };
 
 // Builds the whole launch plan for a LoopTree.

@@ -42,7 +42,18 @@ struct Staged {
   std::map<int, int64_t> scoef;      // var sym -> staged element coefficient
   int64_t cst = 0;                   // staged offset of the all-zero index
   int64_t total = 1;                 // staged elements
+  // k-blocked layout (kblk > 0): k_in = kb * kblk + kr is staged as an
+  // outermost dim kb (stride skb, KB blocks) and an innermost dim kr
+  int64_t kblk = 0, skb = 0, KB = 1;
 };
+
+// staged element offset of inner-dim index `e` (a C expression; multiples of
+// 8 never straddle a block)
+std::string kin_dst(const Staged &s, const std::string &e) {
+  if (!s.kblk) return "(" + e + ")";
+  return "((" + e + ") / " + std::to_string(s.kblk) + "L * " + std::to_string(s.skb) +
+         "L + (" + e + ") % " + std::to_string(s.kblk) + "L)";
+}
 
 int find_inner_dim(const Operand &op, int sym) {
   int found = -1;
@@ -57,13 +68,31 @@ int find_inner_dim(const Operand &op, int sym) {
   return found;
 }
 
-Staged stage_layout(const Operand &op, int inner_dim, int64_t kin_padded) {
+// kblk > 0: the k-blocked layout [kb][dims indexed only by reduced vars]
+// [other dims][kr], so that the rows of one K block are kblk-element runs
+// next to each other (a matmul operand A[M][K] becomes A'[K/kblk][M][kblk], a
+// convolution input X[N][C][H][W] becomes X'[C/kblk][N][H+2][W+2][kblk]).
+Staged stage_layout(const Operand &op, int inner_dim, int64_t kin_padded, int64_t kblk,
+                    const std::vector<VarInfo> &K) {
   Staged s;
   s.input_index = op.input_index;
   s.inner_dim = inner_dim;
   const int nd = (int)op.sizes.size();
+  auto k_only = [&](int d) {
+    if (op.coef[d].empty()) return false;
+    for (const auto &p : op.coef[d]) {
+      bool in_k = false;
+      for (const auto &v : K) in_k = in_k || v.sym == p.first;
+      if (!in_k) return false;
+    }
+    return true;
+  };
+  const bool blocked = kblk > 0 && kin_padded > kblk;
+  if (blocked)
+    for (int d = 0; d < nd; ++d)
+      if (d != inner_dim && k_only(d)) s.order.push_back(d);
   for (int d = 0; d < nd; ++d)
-    if (d != inner_dim) s.order.push_back(d);
+    if (d != inner_dim && !(blocked && k_only(d))) s.order.push_back(d);
   s.order.push_back(inner_dim);
   s.padlo.assign(nd, 0);
   s.ssize.assign(nd, 0);
@@ -79,9 +108,15 @@ Staged stage_layout(const Operand &op, int inner_dim, int64_t kin_padded) {
   for (int q = nd - 1; q >= 0; --q) {
     int d = s.order[q];
     s.sstride[d] = st;
-    st *= s.ssize[d];
+    st *= d == inner_dim && blocked ? kblk : s.ssize[d];
   }
   s.total = st;
+  if (blocked) {
+    s.kblk = kblk;
+    s.KB = kin_padded / kblk;
+    s.skb = st;
+    s.total = st * s.KB;
+  }
   for (int d = 0; d < nd; ++d) {
     for (const auto &p : op.coef[d]) s.scoef[p.first] += p.second * s.sstride[d];
     s.cst += (op.cst[d] + s.padlo[d]) * s.sstride[d];
@@ -222,7 +257,7 @@ int64_t staging_plane_body(std::stringstream &k, const std::string &name, const 
     << "      #pragma unroll\n      for (int j = 0; j < 4; ++j) { v[j] = lo[j]; v[j + 4] = hi[j]; }\n"
     << "    }\n"
     << "    *(T8*)(S + dst0 + (a0 + r) * " << s.sstride[a] << "L + sl * " << s.sstride[ld]
-    << "L + c0 + g * 8) = v;\n  }\n}\n";
+    << "L + " << kin_dst(s, "c0 + g * 8") << ") = v;\n  }\n}\n";
   return nz * nA * nC;
 }
 
@@ -276,7 +311,8 @@ int64_t staging_transpose_body(std::stringstream &k, const std::string &name,
     << "    if (sl >= " << s.ssize[ld] << "L) continue;\n"
     << "    T8 v;\n    #pragma unroll\n    for (int j = 0; j < 8; ++j) v[j] = (" << elt
     << ")tile[g * 8 + j][l];\n"
-    << "    *(T8*)(S + dst0 + sl * " << s.sstride[ld] << "L + c0 + g * 8) = v;\n  }\n}\n";
+    << "    *(T8*)(S + dst0 + sl * " << s.sstride[ld] << "L + " << kin_dst(s, "c0 + g * 8")
+    << ") = v;\n  }\n}\n";
   return gx * gy * nz;
 }
 
@@ -317,7 +353,10 @@ int64_t staging_generic_body(std::stringstream &k, const std::string &name, cons
     if (s.ssize[d] != op.sizes[d])
       ok += " && " + idx + " >= 0 && " + idx + " < " + std::to_string(op.sizes[d]) + "L";
     src += " + " + idx + " * " + std::to_string(op.strides[d]) + "L";
-    dst += " + s" + std::to_string(d) + " * " + std::to_string(s.sstride[d]) + "L";
+    if (d == in)
+      dst += " + " + kin_dst(s, "s" + std::to_string(d));
+    else
+      dst += " + s" + std::to_string(d) + " * " + std::to_string(s.sstride[d]) + "L";
   }
   k << "  const bool ok = " << ok << ";\n"
     << "  T8 v;\n"
@@ -355,6 +394,49 @@ std::string decls(const std::vector<VarInfo> &vs, const std::string &ind) {
   std::string out;
   for (const auto &v : vs) out += ind + "long " + v.name + ";\n";
   return out;
+}
+
+// Halo bound of the Q operand for K tiles of BK over a k-blocked staging
+// whose first outer-K var is the block index and the rest are "taps": 0 when
+// the halo form does not apply (taps that do not shift Q's rows forward, Q
+// rows that are not non-decreasing in j, fewer than two taps), else the
+// number of BK-vectors HL such that every tile's rows at every tap lie within
+// HL vectors of the tile's first row.  ntap: taps per block.
+int64_t halo_rows(const Contraction &c, const Staged &SQ, const std::vector<VarInfo> &KO,
+                  int BN, int BK, int &ntap) {
+  if (KO.size() < 2 || c.ND > (1LL << 24)) return 0;
+  int64_t taps = 1, max_tap = 0;
+  for (size_t q = 1; q < KO.size(); ++q) {
+    taps *= KO[q].size;
+    auto it = SQ.scoef.find(KO[q].sym);
+    const int64_t cf = it == SQ.scoef.end() ? 0 : it->second;
+    if (cf < 0 || cf % BK) return 0;
+    max_tap += cf * (KO[q].size - 1);
+  }
+  if (taps < 2 || taps > (1 << 20)) return 0;
+  // Q row offsets over j (row-major over the J vars, the last fastest)
+  std::vector<int64_t> cf(c.J.size());
+  for (size_t v = 0; v < c.J.size(); ++v) {
+    auto it = SQ.scoef.find(c.J[v].sym);
+    cf[v] = it == SQ.scoef.end() ? 0 : it->second;
+  }
+  std::vector<int64_t> idx(c.J.size(), 0);
+  int64_t prev = 0, first = 0, span = 0;
+  for (int64_t j = 0; j < c.ND; ++j) {
+    int64_t r = 0;
+    for (size_t v = 0; v < c.J.size(); ++v) r += cf[v] * idx[v];
+    if (j > 0 && r < prev) return 0;
+    if (j % BN == 0) first = r;
+    if (r % BK) return 0;
+    span = std::max(span, r - first);
+    prev = r;
+    for (int v = (int)c.J.size() - 1; v >= 0; --v) {  // next j
+      if (++idx[v] < c.J[v].size) break;
+      idx[v] = 0;
+    }
+  }
+  ntap = (int)taps;
+  return (span + max_tap) / BK + 1;
 }
 
 }  // namespace
@@ -414,13 +496,69 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   if (BM % 64 || BN % 64 || BM > 256 || BN > 256 || BM * BN > 256 * 256) return false;
   if (2 * (BM + BN) * (BK + 8) * 2 > 160 * 1024) return false;  // LDS of a CU
   const int64_t KIP = (KI + BK - 1) / BK * BK;
-  Staged st[2] = {stage_layout(c.ops[0], dims[0], KIP), stage_layout(c.ops[1], dims[1], KIP)};
+  // k-blocked staging: K tiles of BK over k_in become separate outer-K
+  // blocks, and a K tile's operand rows are adjacent BK-element runs
+  const bool want_halo = knobs.igemm_halo == 1;
+  const bool blocked = (knobs.stage_blocked != 0 || want_halo) && KIP > BK;
+  const bool kb_first = knobs.kb_outer == 1 || want_halo;
+  Staged st[2] = {stage_layout(c.ops[0], dims[0], KIP, blocked ? BK : 0, c.K),
+                  stage_layout(c.ops[1], dims[1], KIP, blocked ? BK : 0, c.K)};
   const Staged &SP = st[c.po], &SQ = st[c.qo];
-  const int lds = 2 * (BM + BN) * (BK + 8) * 2;
+  int64_t KIT = KIP;  // inner-K extent seen by the tile
+  if (blocked) {
+    // the block index is one more outer-K var: slowest (kb=outer: all
+    // outer-K positions of a block in a row) or fastest
+    const int kb_sym = -1000;
+    VarInfo kb{kb_sym, KIP / BK, "kb_"};
+    if (kb_first) KO.insert(KO.begin(), kb);
+    else KO.push_back(kb);
+    ko_n *= KIP / BK;
+    for (auto &x : st) x.scoef[kb_sym] = x.skb;
+    KIT = BK;
+  }
   const bool full = c.MD % BM == 0 && c.ND % BN == 0;
   const bool int32 = SP.total < (1LL << 31) && SQ.total < (1LL << 31);
   // the pipelined tile addresses each operand with 32-bit byte offsets
   const bool pipe = !knobs.igemm_plain && SP.total < (1LL << 30) && SQ.total < (1LL << 30);
+  // pipelined tile: wave grid (4 waves as 2x2, or 8 as 2x4 / 4x2) and LDS rows
+  int WGM = 2, WGN = 2;
+  bool swz = false;
+  if (pipe) {
+    auto fits = [&](int wgm, int wgn) {
+      const int NT = 64 * wgm * wgn, CPR = BK / 8;
+      return BM % (32 * wgm) == 0 && BN % (32 * wgn) == 0 && BM * CPR % NT == 0 &&
+             BN * CPR % NT == 0 && BM * CPR >= NT && BN * CPR >= NT;
+    };
+    const int waves = knobs.waves > 0 ? knobs.waves : 4;
+    if (waves == 8) {
+      if (BN / 4 >= 32) WGN = 4;
+      else WGM = 4;
+    } else if (waves == 2) {  // two waves of (BM) x (BN/2): fewer LDS reads per MFMA
+      WGM = 1;
+    }
+    if (!fits(WGM, WGN)) WGM = WGN = 2;  // the tile is too small for that wave grid
+    if (!fits(WGM, WGN)) return false;
+    swz = knobs.lds_swz == 1;
+  }
+  // halo form: Q rows of all taps of a K block loaded once per block
+  int64_t HL = 0;
+  int ntap = 0;
+  if (pipe && want_halo && blocked) HL = halo_rows(c, SQ, KO, BN, BK, ntap);
+  if (HL) {
+    // eight waves (2 x 4) where the tile allows, else four (2 x 2)
+    auto fits = [&](int wgm, int wgn) {
+      const int NT = 64 * wgm * wgn;
+      return BM % (32 * wgm) == 0 && BN % (32 * wgn) == 0 && BM * (BK / 8) % NT == 0 &&
+             BM * (BK / 8) >= NT;
+    };
+    WGM = 2;
+    WGN = knobs.waves != 4 && fits(2, 4) ? 4 : 2;
+    if (!fits(WGM, WGN) || 2 * (BM + HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
+  }
+  if (want_halo && !HL) return false;
+  const int lds = HL ? (int)(2 * (BM + HL) * (BK + 8) * 2)
+                     : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
+  const int threads = 64 * WGM * WGN;
 
   // ---- buffers: one staged (16-bit) scratch buffer per operand ----
   int sbuf[2];
@@ -464,8 +602,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   }
   // geometry of the tile
   k << "struct LtGeo {\n"
-    << "  static constexpr int KO = " << ko_n << ", KI = " << KIP << ";\n"
+    << "  static constexpr int KO = " << ko_n << ", KI = " << KIT << ";\n"
     << "  static constexpr long MD = " << c.MD << "L, ND = " << c.ND << "L;\n"
+    << "  static constexpr int NTAP = " << ntap << ";\n"
     << "  long zp, zq, zo;\n"
     << "  __device__ long p_row(long i, bool& ok) const {\n    ok = i < MD;\n"
     << decls(c.I, "    ") << decompose("i", c.I, "    ") << "    return zp + "
@@ -486,7 +625,11 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << decls(c.J, "    ") << decompose("j", c.J, "    ") << "    return "
     << lin_over(c.ostride, c.J) << ";\n  }\n"
     << "};\n";
-  k << "extern \"C\" __global__ void __launch_bounds__(256) lt_igemm0(const " << elt
+  // eight waves: keep two workgroups per CU (four waves per SIMD, at most
+  // 128 VGPRs each) when their LDS fits
+  const bool two_wg = threads == 512 && 2 * lds <= 160 * 1024;
+  k << "extern \"C\" __global__ void __launch_bounds__(" << threads << ")"
+    << (two_wg ? " __attribute__((amdgpu_waves_per_eu(4)))" : "") << " lt_igemm0(const " << elt
     << "* __restrict__ P, const " << elt << "* __restrict__ Q, float* __restrict__ out) {\n"
     << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << lds << "];\n"
     << "  LtGeo g;\n"
@@ -495,28 +638,37 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << "  g.zq = " << lin_over(SQ.scoef, c.B) << ";\n"
     << "  g.zo = " << lin_over(c.ostride, c.B) << ";\n"
     ;
-  if (pipe)
-    k << "  lt_igemm_tile_pipe<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
-      << (full ? "true" : "false") << ">(g, P, Q, out, (long)blockIdx.y * " << BM
-      << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem, "
-      << SP.total * 2 << "u, " << SQ.total * 2 << "u);\n}\n";
+  auto gx = (c.ND + BN - 1) / BN, gy = (c.MD + BM - 1) / BM;
+  ASSERT(gx < (1LL << 31) && gy < 65536 && c.BD < 65536) << "contraction grid too large";
+  const std::string i0 = "(long)blockIdx.y * " + std::to_string(BM);
+  const std::string j0 =
+      "(long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " + std::to_string(BN);
+  const std::string ind = "  ";
+  if (HL)
+    k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL
+      << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2 << "u, "
+      << SQ.total * 2 << "u);\n";
+  else if (pipe)
+    k << ind << "lt_igemm_tile_pipe<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", "
+      << (swz ? "true" : "false") << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, "
+      << SP.total * 2 << "u, " << SQ.total * 2 << "u);\n";
   else
-    k << "  lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
-      << (full ? "true" : "false") << ", " << (int32 ? "int" : "long")
-      << ">(g, P, Q, out, (long)blockIdx.y * " << BM
-      << ", (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " << BN << ", smem);\n}\n";
+    k << ind << "lt_igemm_tile<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << (int32 ? "int" : "long") << ">(g, P, Q, out, "
+      << i0 << ", " << j0 << ", smem);\n";
+  k << "}\n";
 
   Stage g;
   g.kernel = "lt_igemm0";
   g.kind = bf16 ? "contract_bf16" : "contract_f16";
   g.buffers = {sbuf[c.po], sbuf[c.qo], out_buf};
   g.writes = {out_buf};
-  auto gx = (c.ND + BN - 1) / BN, gy = (c.MD + BM - 1) / BM;
-  ASSERT(gx < (1LL << 31) && gy < 65536 && c.BD < 65536) << "contraction grid too large";
   g.grid[0] = (unsigned)gx;
   g.grid[1] = (unsigned)gy;
   g.grid[2] = (unsigned)c.BD;
-  g.block[0] = 256;
+  g.block[0] = (unsigned)threads;
   g.flops = 2 * c.MD * c.ND * c.KD * c.BD;
   g.bytes = SP.total * 2 + SQ.total * 2 + c.MD * c.ND * c.BD * 4;
   plan.stages.push_back(g);
@@ -524,8 +676,14 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   std::stringstream desc;
   desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : "f16")
        << " staged operands, f32 accumulate): D[" << c.BD << "][" << c.MD << "][" << c.ND
-       << "] = P x Q over K=" << ko_n << "x" << KI << (KIP != KI ? " (padded to " + std::to_string(KIP) + ")" : std::string()) << ", tile " << BM << "x" << BN << "x"
-       << BK << (pipe ? " pipelined" : "") << "; staged P " << SP.total << " and Q " << SQ.total
+       << "] = P x Q over K=" << ko_n << "x" << (blocked ? KIT : KI) << (KIP != KI && !blocked ? " (padded to " + std::to_string(KIP) + ")" : std::string()) << ", tile " << BM << "x" << BN << "x"
+       << BK << (pipe ? " pipelined, " + std::to_string(WGM * WGN) + " waves" +
+                            (swz ? ", swizzled LDS" : "")
+                      : std::string())
+       << (blocked ? std::string(", k-blocked staging (kb ") + (kb_first ? "outer)" : "inner)")
+                   : std::string())
+       << (HL ? ", Q halo of " + std::to_string(HL) + " rows per K block" : std::string())
+       << "; staged P " << SP.total << " and Q " << SQ.total
        << " elements\n";
   plan.description = desc.str();
   return true;

@@ -25,6 +25,15 @@
 //                  loader instead of staging them
 //   igemm=plain    staged 16-bit contractions use the single-stage tile
 //                  (default: the software-pipelined tile where it applies)
+//   waves=2|4|8    waves per workgroup of the pipelined tile
+//   lds=swz|pad    LDS rows of the pipelined tile: XOR-permuted 16-byte
+//                  chunks (no padding) or rows padded by 16 bytes
+//   stage=blocked|flat  16-bit contraction operands staged k-blocked
+//                  ([K/BK][...][BK]) or with the whole k_in dim innermost
+//   kb=outer|inner order of the K blocks among the outer-K positions
+//   igemm=halo|pipe  staged 16-bit contractions whose outer-K positions shift
+//                  the Q rows (convolution taps) load those rows once per
+//                  K block into an LDS halo (halo) or per K tile (pipe)
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -169,6 +178,15 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       else if (tok == "mfma_f16") k.f16 = true;
       else if (tok == "contract=gather") k.gather = true;
       else if (tok == "igemm=plain") k.igemm_plain = true;
+      else if (tok == "igemm=halo") k.igemm_halo = 1;
+      else if (tok == "igemm=pipe") k.igemm_halo = 0;
+      else if (tok == "lds=swz") k.lds_swz = 1;
+      else if (tok == "lds=pad") k.lds_swz = 0;
+      else if (tok == "stage=blocked") k.stage_blocked = 1;
+      else if (tok == "stage=flat") k.stage_blocked = 0;
+      else if (tok == "kb=outer") k.kb_outer = 1;
+      else if (tok == "kb=inner") k.kb_outer = 0;
+      else if (tok.rfind("waves=", 0) == 0) k.waves = std::stoi(tok.substr(6));
       else if (tok.rfind("bk=", 0) == 0) k.bk = std::stoi(tok.substr(3));
       else if (tok.rfind("block=", 0) == 0) k.block = std::stoi(tok.substr(6));
       else if (tok.rfind("vec=", 0) == 0) k.vec = std::stoi(tok.substr(4));
@@ -188,6 +206,8 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
   }
   ASSERT(k.block >= 64 && k.block <= 1024 && k.block % 64 == 0)
       << "block=N must be a multiple of 64 in [64, 1024]";
+  ASSERT(k.waves == 0 || k.waves == 2 || k.waves == 4 || k.waves == 8)
+      << "waves=N must be 2, 4 or 8";
   return k;
 }
 

@@ -188,6 +188,9 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 
 // Software-pipelined form of the same tile, for staged operands of fewer than
 // 2^31 bytes with KI % BK == 0 (pbytes, qbytes: operand sizes in bytes).
+//  * WGM x WGN waves (4 or 8) split the tile; each wave owns
+//    (BM/WGM) x (BN/WGN) as 32x32 MFMA blocks.  Eight waves put two waves on
+//    every SIMD, so one wave's MFMAs cover the other's LDS and load waits.
 //  * Global loads are buffer loads: a 32-bit byte offset per row in a VGPR
 //    and the K tile's offset in an SGPR; rows outside the problem get an
 //    offset beyond the buffer, which the hardware reads as 0.
@@ -198,44 +201,73 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 //    with the last step's MFMAs; one barrier per K tile.
 //  * sched_group_barrier pins the interleave in the steady state: one MFMA,
 //    then one fragment read and one global load (or one LDS store).
+//  * LDS rows are either padded by 16 bytes (SWZ false) or unpadded with the
+//    16-byte chunks of each row XOR-permuted (SWZ true): chunk c of row r is
+//    stored at chunk c ^ ((r / RP) % CPR), RP = rows per 256 bytes (the 64
+//    four-byte banks of the gfx950 LDS).  Any 16 consecutive lanes of a
+//    fragment read (16 rows, one logical chunk) and of a tile store (whole
+//    rows) then touch 16 distinct 16-byte bank groups.
 // Each lane's working set: TM*TN*16 accumulators, 2*(TM+TN) fragments of 4
 // VGPRs and 2*(VP+VQ) load registers of 4.
-template <typename T, int BM, int BN, int BK, bool FULL, class G>
+template <int BM, int BN, int BK, bool SWZ>
+__host__ __device__ constexpr int lt_igemm_pipe_lds_bytes() {
+  return 2 * (BM + BN) * (SWZ ? BK : BK + 8) * 2;
+}
+
+template <int BK, bool SWZ>
+__device__ __forceinline__ int lt_ig_lds_off(int row, int chunk) {
+  constexpr int CPR = BK / 8;
+  if (SWZ) {
+    constexpr int RP = 128 / BK;  // rows per 256 bytes: 4 (BK 32) or 2 (BK 64)
+    return row * BK + ((chunk ^ ((row / RP) % CPR)) << 3);
+  }
+  return row * (BK + 8) + (chunk << 3);
+}
+
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool SWZ, class G>
 __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restrict__ P,
                                                    const T* __restrict__ Q,
                                                    float* __restrict__ out, long i0, long j0,
                                                    unsigned char* smem, unsigned pbytes,
                                                    unsigned qbytes) {
   typedef typename lt_ig_vec8<T>::type V;
-  constexpr int NT = 256, LDR = BK + 8;
-  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 32, TN = WN / 32;
+  constexpr int NT = 64 * WGM * WGN, LDR = SWZ ? BK : BK + 8;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
   constexpr int CPR = BK / 8, KK = BK / 16;
   constexpr int VP = BM * CPR / NT, VQ = BN * CPR / NT;
+  static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
   static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
-                "tile too small for 256 lanes");
+                "tile too small for the workgroup");
   static_assert(KK % 2 == 0, "BK must be a multiple of 32");
   const int KIT = g.KI / BK;
   const int KT = g.KO * KIT;
   T* sP = (T*)smem;              // [2][BM][LDR]
   T* sQ = sP + 2 * BM * LDR;     // [2][BN][LDR]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int cc = (tid % CPR) * 8;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int ck = tid % CPR, cc = ck * 8;
   const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
   const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
   unsigned pv[VP], qv[VQ];
+  int sp[VP], sq[VQ];  // LDS element offsets of this lane's tile stores
 #pragma unroll
   for (int v = 0; v < VP; ++v) {
     bool ok;
-    const long o = g.p_row(i0 + (tid + v * NT) / CPR, ok);
+    const int r = (tid + v * NT) / CPR;
+    const long o = g.p_row(i0 + r, ok);
     pv[v] = (FULL || ok) ? (unsigned)((o + cc) * 2) : 0x80000000u;
+    sp[v] = lt_ig_lds_off<BK, SWZ>(r, ck);
   }
 #pragma unroll
   for (int v = 0; v < VQ; ++v) {
     bool ok;
-    const long o = g.q_row(j0 + (tid + v * NT) / CPR, ok);
+    const int r = (tid + v * NT) / CPR;
+    const long o = g.q_row(j0 + r, ok);
     qv[v] = (FULL || ok) ? (unsigned)((o + cc) * 2) : 0x80000000u;
+    sq[v] = lt_ig_lds_off<BK, SWZ>(r, ck);
   }
+  // fragment rows of this lane; the chunk of k16 step kk is 2 * kk + lane / 32
+  const int fr = lane & 31, fh = lane >> 5;
   lt_ig_u32x4 r0p[VP], r0q[VQ], r1p[VP], r1q[VQ];
   auto load = [&](int kt, lt_ig_u32x4* rp, lt_ig_u32x4* rq) {
     const int ko = kt / KIT, kb = (kt % KIT) * BK;
@@ -249,22 +281,20 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
   };
   auto store = [&](int buf, const lt_ig_u32x4* rp, const lt_ig_u32x4* rq) {
 #pragma unroll
-    for (int v = 0; v < VP; ++v)
-      *(lt_ig_u32x4*)(sP + (buf * BM + (tid + v * NT) / CPR) * LDR + cc) = rp[v];
+    for (int v = 0; v < VP; ++v) *(lt_ig_u32x4*)(sP + buf * BM * LDR + sp[v]) = rp[v];
 #pragma unroll
-    for (int v = 0; v < VQ; ++v)
-      *(lt_ig_u32x4*)(sQ + (buf * BN + (tid + v * NT) / CPR) * LDR + cc) = rq[v];
+    for (int v = 0; v < VQ; ++v) *(lt_ig_u32x4*)(sQ + buf * BN * LDR + sq[v]) = rq[v];
   };
   V fa[2][TM], fb[2][TN];
   auto frags = [&](int buf, int kk, int s) {
 #pragma unroll
     for (int a = 0; a < TM; ++a)
-      fa[s][a] = *(const V*)(sP + (buf * BM + wm * WM + a * 32 + (lane & 31)) * LDR +
-                             kk * 16 + (lane >> 5) * 8);
+      fa[s][a] = *(const V*)(sP + buf * BM * LDR +
+                             lt_ig_lds_off<BK, SWZ>(wm * WM + a * 32 + fr, 2 * kk + fh));
 #pragma unroll
     for (int b = 0; b < TN; ++b)
-      fb[s][b] = *(const V*)(sQ + (buf * BN + wn * WN + b * 32 + (lane & 31)) * LDR +
-                             kk * 16 + (lane >> 5) * 8);
+      fb[s][b] = *(const V*)(sQ + buf * BN * LDR +
+                             lt_ig_lds_off<BK, SWZ>(wn * WN + b * 32 + fr, 2 * kk + fh));
   };
   lt_ig_f32x16 acc[TM][TN];
 #pragma unroll
@@ -330,6 +360,176 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
     step(kt + 1, 1, r0p, r0q, r1p, r1q, false, false);
   } else {
     step(kt, 0, r1p, r1q, r0p, r0q, false, false);
+  }
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+}
+
+// Halo form of the pipelined tile, for a k-blocked staged Q whose outer-K
+// positions within a K block ("taps", g.NTAP of them) only shift the row
+// offset: Q(j, kb, tap, kr) = q_row(j) + q_koff(kb * NTAP + tap) + kr with
+// q_row non-decreasing in j and q_koff(kb * NTAP + tap) - q_koff(kb * NTAP)
+// a non-negative multiple of BK (a convolution's window taps over a
+// position-major staged input X'[C/BK][N][H+2][W+2][BK]).  K tiles run block
+// by block, taps inner.  The Q rows all taps of one block need are the HL
+// consecutive BK-vectors starting at the tile's first row: they are loaded
+// once per block into an LDS halo buffer (double-buffered; the next block's
+// halo is loaded during the first tap and stored to LDS after the last) and
+// every tap reads its fragments from it at a uniform shift, instead of
+// reloading the tile's rows from L2 for every tap.  P goes through the same
+// two-tiles-ahead register pipeline and double-buffered LDS as in
+// lt_igemm_tile_pipe.  HL (a host-computed bound over all tiles) must cover
+// (q_row(j_last) - q_row(j0) + max tap shift) / BK + 1.
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, class G>
+__device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restrict__ P,
+                                                   const T* __restrict__ Q,
+                                                   float* __restrict__ out, long i0, long j0,
+                                                   unsigned char* smem, unsigned pbytes,
+                                                   unsigned qbytes) {
+  typedef typename lt_ig_vec8<T>::type V;
+  constexpr int NT = 64 * WGM * WGN, LDR = BK + 8;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+  constexpr int CPR = BK / 8, KK = BK / 16;
+  constexpr int VP = BM * CPR / NT;
+  constexpr int HC = HL * CPR, HV = (HC + NT - 1) / NT;  // halo 16-byte chunks
+  static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
+  static_assert(VP >= 1 && BM * CPR % NT == 0, "tile too small for the workgroup");
+  static_assert(KK % 2 == 0, "BK must be a multiple of 32");
+  constexpr int NTAP = G::NTAP;
+  static_assert(NTAP >= 2, "the halo form needs at least two taps per K block");
+  const int KT = g.KO, KB = KT / NTAP;
+  T* sP = (T*)smem;              // [2][BM][LDR]
+  T* sH = sP + 2 * BM * LDR;     // [2][HL][LDR]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int ck = tid % CPR, cc = ck * 8;
+  const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
+  const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
+  unsigned pv[VP];
+  int sp[VP];
+#pragma unroll
+  for (int v = 0; v < VP; ++v) {
+    bool ok;
+    const int r = (tid + v * NT) / CPR;
+    const long o = g.p_row(i0 + r, ok);
+    pv[v] = (FULL || ok) ? (unsigned)((o + cc) * 2) : 0x80000000u;
+    sp[v] = r * LDR + cc;
+  }
+  // the tile's first Q row and each fragment row's vector offset from it
+  // (rows past the problem reuse the last row: their results are not stored)
+  bool ok0;
+  const long qb0 = g.q_row(j0, ok0);
+  const int fr = lane & 31, fh = lane >> 5;
+  int lq[TN];
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    long j = j0 + wn * WN + b * 32 + fr;
+    if (!FULL && j >= g.ND) j = g.ND - 1;
+    bool ok;
+    lq[b] = (int)((g.q_row(j, ok) - qb0) / BK) * LDR + fh * 8;
+  }
+  // halo chunks of this lane: chunk e = tid + h * NT is vector e / CPR, part e % CPR
+  unsigned hv[HV];
+#pragma unroll
+  for (int h = 0; h < HV; ++h) {
+    const int e = tid + h * NT;
+    hv[h] = e < HC ? (unsigned)((qb0 + (long)(e / CPR) * BK + (e % CPR) * 8) * 2) : 0x80000000u;
+  }
+  lt_ig_u32x4 hr[HV];
+  // the block offset goes into the per-lane offset, which the buffer range
+  // check covers: a halo tail past the end of Q reads zeros
+  auto hload = [&](int kb) {
+    const unsigned bo = (unsigned)(g.q_koff(kb * NTAP) * 2);
+#pragma unroll
+    for (int h = 0; h < HV; ++h)
+      hr[h] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)(hv[h] + bo), 0, 0);
+  };
+  auto hstore = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < HV; ++h) {
+      const int e = tid + h * NT;
+      if (HC % NT == 0 || e < HC)
+        *(lt_ig_u32x4*)(sH + buf * HL * LDR + (e / CPR) * LDR + (e % CPR) * 8) = hr[h];
+    }
+  };
+  lt_ig_u32x4 r0p[VP], r1p[VP];
+  auto load = [&](int kt, lt_ig_u32x4* rp) {
+    const int po = (int)(g.p_koff(kt) * 2);
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      rp[v] = __builtin_amdgcn_raw_buffer_load_b128(rsP, (int)pv[v], po, 0);
+  };
+  auto store = [&](int buf, const lt_ig_u32x4* rp) {
+#pragma unroll
+    for (int v = 0; v < VP; ++v) *(lt_ig_u32x4*)(sP + buf * BM * LDR + sp[v]) = rp[v];
+  };
+  // LDS element offset of tile kt's tap shift within its block's halo buffer
+  auto hoff = [&](int kt) {
+    const int kb = kt / NTAP;
+    return (kb & 1) * HL * LDR + (int)((g.q_koff(kt) - g.q_koff(kb * NTAP)) / BK) * LDR;
+  };
+  V fa[2][TM], fb[2][TN];
+  auto frags = [&](int buf, int ho, int kk, int s) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[s][a] = *(const V*)(sP + buf * BM * LDR + (wm * WM + a * 32 + fr) * LDR + kk * 16 +
+                             fh * 8);
+#pragma unroll
+    for (int b = 0; b < TN; ++b) fb[s][b] = *(const V*)(sH + ho + lq[b] + kk * 16);
+  };
+  lt_ig_f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  // One K tile kt: its P tile is in LDS buffer B, its block's halo in halo
+  // buffer (kt / NTAP) % 2, its k16-step-0 fragments in fa/fb[0]; RN holds P
+  // of tile kt+1, RF receives P of tile kt+2 when ld.  With st, tile kt+1's P
+  // goes to buffer B^1 and its first fragments are read after the barrier.
+  auto step = [&](int kt, int B, const lt_ig_u32x4* RN, lt_ig_u32x4* RF, bool ld, bool st) {
+    const int kb = kt / NTAP, tap = kt - kb * NTAP;
+    const int ho = hoff(kt);
+    if (ld) load(kt + 2, RF);
+    const bool next_halo = kb + 1 < KB;
+    if (tap == 0 && next_halo) hload(kb + 1);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const int s = kk & 1;
+      if (kk + 1 < KK) frags(B, ho, kk + 1, s ^ 1);
+      if (kk == KK - 1 && st) store(B ^ 1, RN);
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
+    }
+    if (tap == NTAP - 1 && next_halo) hstore((kb + 1) & 1);
+    if (st) {
+      __syncthreads();
+      frags(B ^ 1, hoff(kt + 1), 0, 0);
+    }
+  };
+  hload(0);
+  load(0, r0p);
+  if (KT > 1) load(1, r1p);
+  hstore(0);
+  store(0, r0p);
+  __syncthreads();
+  frags(0, hoff(0), 0, 0);
+  // tile t's P lives in register set t % 2 and LDS buffer t % 2
+  int kt = 0;
+  for (; kt + 3 < KT; kt += 2) {
+    step(kt, 0, r1p, r0p, true, true);
+    step(kt + 1, 1, r0p, r1p, true, true);
+  }
+  const int R = KT - kt;  // 1, 2 or 3 tiles left; kt is even
+  if (R == 3) {
+    step(kt, 0, r1p, r0p, true, true);
+    step(kt + 1, 1, r0p, r1p, false, true);
+    step(kt + 2, 0, r1p, r0p, false, false);
+  } else if (R == 2) {
+    step(kt, 0, r1p, r0p, false, true);
+    step(kt + 1, 1, r0p, r1p, false, false);
+  } else {
+    step(kt, 0, r1p, r0p, false, false);
   }
   lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
 }

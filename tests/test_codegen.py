@@ -66,6 +66,13 @@ def test_staged_contraction_plans(lt, tile):
 
 def test_staged_layout_of_conv_operands(lt):
     out, _ = P.conv2d(lt, 32, 128, 128, 56, 56)
-    d = lt.compile(out.loop_tree, "hip_bf16").dump()
-    # X'[32][58][58][128] and W'[128][3][3][128], K = 9 x 128
+    t = out.loop_tree
+    d = lt.compile(t, "hip_bf16").dump()
+    # k-blocked by the K tile of 32: X'[4][32][58][58][32] and W'[4][3][3][128][32],
+    # K = (4 blocks x 9 taps) x 32
+    assert "K=36x32" in d and "k-blocked staging" in d
+    assert "staged P 147456 and Q 13778944 elements" in d
+    # flat: X'[32][58][58][128] and W'[128][3][3][128], K = 9 x 128
+    d = lt.compile(t.annotate(t.roots[0], "stage=flat"), "hip_bf16").dump()
     assert "K=9x128" in d and "staged P 147456 and Q 13778944 elements" in d
+    assert "k-blocked" not in d.splitlines()[0]

@@ -147,11 +147,15 @@ STAGED = CONTRACT
 @pytest.mark.parametrize("name,kw", STAGED, ids=["%s%d" % (c[0], i) for i, c in enumerate(STAGED)])
 @pytest.mark.parametrize("tile", ["", "tile=64x64", "tile=128x64", "tile=64x128", "tile=128x256",
                                   "tile=128x256 bk=64", "tile=64x64 bk=64", "igemm=plain",
-                                  "tile=128x256 bk=64 igemm=plain", "contract=gather"])
+                                  "tile=128x256 bk=64 igemm=plain", "contract=gather",
+                                  "stage=flat", "waves=8", "waves=2", "lds=swz",
+                                  "tile=256x256 bk=64 waves=8 lds=swz", "kb=outer"])
 def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
-    """Staged-operand lowering (both operands permuted, zero-haloed and rounded
-    to bf16 by one staging launch, then the implicit-GEMM tile, pipelined or
-    single-stage) at every tile shape, and the gathered lowering, against float64 of the bf16-rounded operands."""
+    """Staged-operand lowering (both operands permuted -- k-blocked or flat --,
+    zero-haloed and rounded to bf16 by one staging launch, then the
+    implicit-GEMM tile, pipelined or single-stage, with 2, 4 or 8 waves and
+    padded or XOR-permuted LDS rows) at every tile shape, and the gathered
+    lowering, against float64 of the bf16-rounded operands."""
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     t = out.loop_tree
@@ -172,3 +176,45 @@ def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
     e_bf = normwise(y, f64_bf)
     print(name, kw, tile, "hip-vs-f64(bf16 operands) %.3g" % e_bf)
     assert e_bf <= 1e-5
+
+
+HALO = [
+    ("conv2d", dict(N=4, Ci=64, Co=128, H=28, W=28)),
+    ("conv2d", dict(N=3, Ci=128, Co=40, H=14, W=14)),
+    ("conv2d", dict(N=2, Ci=64, Co=72, H=15, W=15, stride=2)),
+    ("conv2d", dict(N=1, Ci=64, Co=8, H=9, W=12, pad=2, k=5)),
+    ("conv2d", dict(N=1, Ci=40, Co=24, H=5, W=300)),
+    ("conv2d", dict(N=2, Ci=96, Co=64, H=7, W=7, pad=0)),
+]
+
+
+@pytest.mark.parametrize("name,kw", HALO, ids=["%s%d" % (c[0], i) for i, c in enumerate(HALO)])
+@pytest.mark.parametrize("tile", ["", "tile=128x128", "tile=64x128", "tile=128x256 waves=4"])
+def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
+    """The halo form of the staged tile (the Q rows of all window taps of a
+    channel block loaded once into LDS) computes the same K order as the
+    pipelined tile over the same k-blocked staging (blocks outer, taps
+    inner): the two results must be bit-identical, and within float32
+    accumulation error of float64 over the bf16-rounded operands.  Covers
+    partial tiles, stride 2, a 5x5 window, rows longer than a tile, pad 0
+    and channel counts that are not a multiple of the K block."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    ys = []
+    for knobs in ("igemm=halo", "stage=blocked kb=outer"):
+        t = out.loop_tree
+        t = t.annotate(t.roots[0], (knobs + " " + tile).strip())
+        cc = lt.compile(t, "hip_bf16")
+        assert cc.string_property("kinds") == "stage,contract_bf16"
+        assert ("Q halo" in cc.dump()) == (knobs == "igemm=halo")
+        y = np.zeros(t.io_numel()[len(inputs)], np.float32)
+        cc.run(inputs + [y])
+        ys.append(y)
+    assert np.array_equal(ys[0].view(np.uint32), ys[1].view(np.uint32))
+
+    def bf16(a):
+        u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+        return u.astype(np.uint32).view(np.float32)
+    f64_bf = P.reference_output(name, [bf16(a) for a in inputs], **ref_kw(name, kw)).ravel()
+    assert normwise(ys[0], f64_bf) <= 1e-5

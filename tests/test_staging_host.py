@@ -109,8 +109,11 @@ def staged_weights(Wt, kip):
     return out
 
 
-def run_staging(lt, out, inputs, tmp):
-    cc = lt.compile(out.loop_tree, "hip_bf16")
+def run_staging(lt, out, inputs, tmp, annotate=""):
+    t = out.loop_tree
+    if annotate:
+        t = t.annotate(t.roots[0], annotate)
+    cc = lt.compile(t, "hip_bf16")
     assert cc.string_property("kinds") == "stage,contract_bf16"
     src = cc.string_property("code")
     blocks = re.search(r"static constexpr unsigned lt_stage_blocks\[2\] = \{[^}]*\};", src).group(0)
@@ -159,7 +162,7 @@ def test_conv_staging_on_host(lt, kw, tmpdir_):
     pad, k, stride = kw.get("pad", 1), kw.get("k", 3), kw.get("stride", 1)
     out, shapes = P.conv2d(lt, **kw)
     X, Wt = P.uniform(7, shapes[0]), P.uniform(8, shapes[1])
-    cc = run_staging(lt, out, [X, Wt], tmpdir_)
+    cc = run_staging(lt, out, [X, Wt], tmpdir_, "stage=flat")
     d = cc.dump()
     kin = kw["Ci"]
     m = re.search(r"K=\d+x%d(?: \(padded to (\d+)\))?" % kin, d)
@@ -176,7 +179,7 @@ def test_matmul_staging_on_host(lt, M, N, K, tmpdir_):
     """A[M][K] rounded in place (K padded), B[K][N] transposed to Bt[N][K]."""
     out, shapes = P.matmul(lt, M, N, K)
     A, B = P.uniform(9, shapes[0]), P.uniform(10, shapes[1])
-    cc = run_staging(lt, out, [A, B], tmpdir_)
+    cc = run_staging(lt, out, [A, B], tmpdir_, "stage=flat")
     m = re.search(r"K=1x%d(?: \(padded to (\d+)\))?" % K, cc.dump())
     kip = int(m.group(1)) if m.group(1) else K
     ea = np.zeros((M, kip), np.uint16)
@@ -186,3 +189,54 @@ def test_matmul_staging_on_host(lt, M, N, K, tmpdir_):
     A.tofile(os.path.join(tmpdir_, "in0.bin"))
     B.tofile(os.path.join(tmpdir_, "in1.bin"))
     check_staged(tmpdir_, [ea, eb], [A.size, B.size])
+
+
+def blocked(flat, bk, row_axes):
+    """flat staged layout [...][kip] -> k-blocked [kip/bk][row_axes order][bk]."""
+    kb = flat.shape[-1] // bk
+    b = flat.reshape(flat.shape[:-1] + (kb, bk))
+    nd = flat.ndim - 1
+    return b.transpose((nd,) + tuple(row_axes) + (nd + 1,))
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("kw", [CONV[0], CONV[1], CONV[2], CONV[3]], ids=["0", "1", "2", "3"])
+@pytest.mark.parametrize("bk", [32, 64])
+def test_conv_blocked_staging_on_host(lt, kw, bk, tmpdir_):
+    """stage=blocked (the default where k_in spans several K tiles):
+    X'[C/bk][N][H+2][W+2][bk] and W'[C/bk][k][k][Co][bk] (the window dims,
+    indexed by reduced vars only, before the output channels)."""
+    pad, k, stride = kw.get("pad", 1), kw.get("k", 3), kw.get("stride", 1)
+    out, shapes = P.conv2d(lt, **kw)
+    X, Wt = P.uniform(7, shapes[0]), P.uniform(8, shapes[1])
+    cc = run_staging(lt, out, [X, Wt], tmpdir_, "stage=blocked bk=%d" % bk)
+    kin = kw["Ci"]
+    kip = -(-kin // bk) * bk
+    d = cc.dump().splitlines()[0]  # the plan description
+    if kip == bk:  # a single K block: the flat layout
+        assert "k-blocked" not in d
+        exp = [staged_conv_input(X, pad, k, stride, kip), staged_weights(Wt, kip)]
+    else:
+        assert "k-blocked" in d
+        exp = [blocked(staged_conv_input(X, pad, k, stride, kip), bk, (0, 1, 2)),
+               blocked(staged_weights(Wt, kip), bk, (1, 2, 0))]
+    X.tofile(os.path.join(tmpdir_, "in0.bin"))
+    Wt.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, exp, [X.size, Wt.size])
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("M,N,K", [(64, 96, 128), (300, 200, 704)])
+def test_matmul_blocked_staging_on_host(lt, M, N, K, tmpdir_):
+    """stage=blocked: A'[K/32][M][32] and Bt'[K/32][N][32]."""
+    out, shapes = P.matmul(lt, M, N, K)
+    A, B = P.uniform(9, shapes[0]), P.uniform(10, shapes[1])
+    cc = run_staging(lt, out, [A, B], tmpdir_, "stage=blocked bk=32")
+    kip = -(-K // 32) * 32
+    ea = np.zeros((M, kip), np.uint16)
+    ea[:, :K] = bf16_bits(A)
+    eb = np.zeros((N, kip), np.uint16)
+    eb[:, :K] = bf16_bits(B.T)
+    A.tofile(os.path.join(tmpdir_, "in0.bin"))
+    B.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, [blocked(ea, 32, (0,)), blocked(eb, 32, (0,))], [A.size, B.size])
