@@ -26,7 +26,7 @@ MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f32": 157.3}
 # HBM bytes per launch of the dominant kernel, from FETCH_SIZE / WRITE_SIZE
 # counter runs scaled by a copy kernel of known size
 # (profiles/lt_row0_softmax_8192x8192_hbm_traffic.md)
-MEASURED_TRAFFIC = {"softmax": 537468518}
+MEASURED_TRAFFIC = {"softmax": 537084416}
 
 
 def parse():

@@ -113,8 +113,11 @@ bool try_rowreduce(const Dataflow &df, const Knobs &knobs, Plan &plan) {
   if (vec != 1 && vec != 2 && vec != 4) vec = 1;
   if (C % vec) vec = 1;
   int T = knobs.block;
-  if (T == 256) {  // default: up to 32 values per lane
-    int64_t want = (C + vec * 8 - 1) / (vec * 8);
+  if (T == 256) {
+    // default: two vectors (8 values) per lane, as many lanes as that needs
+    // up to 1024 (measured fastest for rows of 1K-8K floats: more rows in
+    // flight per CU hide the two reductions' barriers)
+    int64_t want = (C + vec * 2 - 1) / (vec * 2);
     T = 64;
     while (T < 1024 && T < want) T *= 2;
   }
