@@ -479,7 +479,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     BN = knobs.tile_n;
     BK = knobs.bk > 0 ? knobs.bk : 32;
   } else {
-    const int cand[][3] = {{256, 256, 64}, {128, 256, 32}, {256, 128, 32}, {128, 128, 64},
+    const int cand[][3] = {{256, 256, 32}, {128, 256, 32}, {256, 128, 32}, {128, 128, 64},
                            {128, 64, 32},  {64, 128, 32},  {64, 64, 32}};
     for (auto &t : cand) {
       if (t[0] > ((c.MD + 63) / 64) * 64 || t[1] > ((c.ND + 63) / 64) * 64) continue;
@@ -529,7 +529,10 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
       return BM % (32 * wgm) == 0 && BN % (32 * wgn) == 0 && BM * CPR % NT == 0 &&
              BN * CPR % NT == 0 && BM * CPR >= NT && BN * CPR >= NT;
     };
-    const int waves = knobs.waves > 0 ? knobs.waves : 4;
+    // default: 8 waves for a 256 x 256 tile (128 accumulators per lane
+    // instead of 256, so the two register sets of the load pipeline fit
+    // without spilling), else 4
+    const int waves = knobs.waves > 0 ? knobs.waves : (BM * BN == 256 * 256 ? 8 : 4);
     if (waves == 8) {
       if (BN / 4 >= 32) WGN = 4;
       else WGM = 4;
@@ -626,8 +629,10 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << lin_over(c.ostride, c.J) << ";\n  }\n"
     << "};\n";
   // eight waves: keep two workgroups per CU (four waves per SIMD, at most
-  // 128 VGPRs each) when their LDS fits
-  const bool two_wg = threads == 512 && 2 * lds <= 160 * 1024;
+  // 128 VGPRs each) when their LDS fits and a wave's accumulators take at
+  // most 64 of those registers
+  const bool two_wg = threads == 512 && 2 * lds <= 160 * 1024 &&
+                      (BM / WGM) * (BN / WGN) / 64 <= 64;
   k << "extern \"C\" __global__ void __launch_bounds__(" << threads << ")"
     << (two_wg ? " __attribute__((amdgpu_waves_per_eu(4)))" : "") << " lt_igemm0(const " << elt
     << "* __restrict__ P, const " << elt << "* __restrict__ Q, float* __restrict__ out) {\n"
