@@ -63,8 +63,10 @@ def knob_candidates(kind):
     if kind == "rowreduce":
         return ["block=%d vec=%d" % (b, v) for b, v in
                 itertools.product((64, 128, 256, 512, 1024), (4, 2, 1))]
-    if kind == "contract_f32":
-        return ["tile=%dx%d" % t for t in ((64, 64), (128, 64), (64, 128), (128, 128))]
+    if kind == "contract_f32":  # staged float tile, or the gathered tile
+        return ["tile=%dx%d" % t for t in ((64, 64), (128, 64), (64, 128), (128, 128))] + [
+            "contract=staged", "contract=staged tile=128x64", "contract=staged waves=8",
+            "contract=gather"]
     if kind.startswith("contract"):  # staged 16-bit operands
         tiles = ["tile=128x256", "tile=256x128", "tile=128x128 bk=64", "tile=256x256 bk=64",
                  "tile=64x128", "tile=128x64"]

@@ -46,8 +46,10 @@ struct Knobs {
   int vec = 0;                 // "vec=N" elements per thread (0 = auto)
   int tile_m = 0, tile_n = 0;  // "tile=MxN" for contractions (0 = auto)
   int bk = 0;                  // "bk=N": K tile of 16-bit contractions (32, 64)
-  bool gather = false;         // "contract=gather": 16-bit contractions without
+  bool gather = false;         // "contract=gather": contractions without
                                // operand staging
+  bool staged = false;         // "contract=staged": float contractions staged
+                               // whatever their size
   bool igemm_plain = false;    // "igemm=plain": the single-stage tile instead of
                                // the software-pipelined one
   int waves = 0;               // "waves=2|4|8": waves per workgroup of the

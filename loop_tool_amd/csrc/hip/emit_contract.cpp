@@ -178,8 +178,9 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
   for (auto &v : K) KD *= v.size;
   for (auto &v : B) BD *= v.size;
 
-  // 16-bit operands: staged operands and the implicit-GEMM tile (emit_igemm.cpp)
-  if ((knobs.bf16 || knobs.f16) && !knobs.gather) {
+  // staged operands and the implicit-GEMM tile (emit_igemm.cpp): 16-bit
+  // operands, and float operands where the pipelined float tile applies
+  if (!knobs.gather) {
     Contraction c;
     c.ops[0] = ops[0];
     c.ops[1] = ops[1];

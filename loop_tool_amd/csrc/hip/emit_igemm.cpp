@@ -178,7 +178,10 @@ void fixed_dims(std::stringstream &k, const Operand &op, const Staged &s,
 int64_t staging_plane_body(std::stringstream &k, const std::string &name, const Operand &op,
                            const Staged &s, int ld, int a, int64_t kin, const char *elt) {
   const int nd = (int)op.sizes.size(), in = s.inner_dim;
-  const int TI = s.ssize[in] % 64 == 0 ? 64 : 32, TP = TI + 4;
+  // inner values per workgroup: 64 16-bit or 32 float values (the LDS tile
+  // holds NP positions of TI values)
+  const bool wide = std::string(elt) == "float";
+  const int TI = s.ssize[in] % 64 == 0 && !wide ? 64 : 32, TP = TI + 4;
   const int64_t L = op.sizes[ld];
   const int64_t RA = std::min<int64_t>(s.ssize[a], std::max<int64_t>(1, 256 / L));
   const int64_t NP = RA * L, NQ = (NP + 3) / 4, NQP = (NQ + 7) / 8 * 8, CP = TI / 2;
@@ -443,8 +446,12 @@ int64_t halo_rows(const Contraction &c, const Staged &SQ, const std::vector<VarI
 
 bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &knobs,
                          Plan &plan) {
-  const bool bf16 = knobs.bf16;
-  const char *elt = bf16 ? "__bf16" : "_Float16";
+  const bool bf16 = knobs.bf16, f32 = !knobs.bf16 && !knobs.f16;
+  const char *elt = bf16 ? "__bf16" : f32 ? "float" : "_Float16";
+  const int EB = f32 ? 4 : 2;  // bytes per staged element
+  // float operands: staging pays for itself on large nests only (small ones
+  // stay on the single-launch gathered tile unless "contract=staged")
+  if (f32 && !knobs.staged && c.MD * c.ND * c.KD * c.BD < (1LL << 28)) return false;
   // k_in: the largest reduced var that indexes one dim of each operand alone
   int kin = -1, dims[2] = {-1, -1};
   for (size_t q = 0; q < c.K.size(); ++q) {
@@ -479,9 +486,13 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     BN = knobs.tile_n;
     BK = knobs.bk > 0 ? knobs.bk : 32;
   } else {
-    const int cand[][3] = {{256, 256, 32}, {128, 256, 32}, {256, 128, 32}, {128, 128, 64},
-                           {128, 64, 32},  {64, 128, 32},  {64, 64, 32}};
-    for (auto &t : cand) {
+    // 16-bit: up to 256 x 256; float: up to 128 x 128 (its fragments and
+    // loads take twice the registers per k)
+    const int cand16[][3] = {{256, 256, 32}, {128, 256, 32}, {256, 128, 32}, {128, 128, 64},
+                             {128, 64, 32},  {64, 128, 32},  {64, 64, 32}};
+    const int cand32[][3] = {{128, 128, 32}, {128, 64, 32}, {64, 128, 32}, {64, 64, 32},
+                             {64, 64, 32},   {64, 64, 32},  {64, 64, 32}};
+    for (auto &t : f32 ? cand32 : cand16) {
       if (t[0] > ((c.MD + 63) / 64) * 64 || t[1] > ((c.ND + 63) / 64) * 64) continue;
       if (tiles(t[0], t[1]) >= 256) {
         BM = t[0];
@@ -494,7 +505,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   }
   if (BK != 32 && BK != 64) return false;
   if (BM % 64 || BN % 64 || BM > 256 || BN > 256 || BM * BN > 256 * 256) return false;
-  if (2 * (BM + BN) * (BK + 8) * 2 > 160 * 1024) return false;  // LDS of a CU
+  if (2 * (BM + BN) * (BK + 16 / EB) * EB > 160 * 1024) return false;  // LDS of a CU
   const int64_t KIP = (KI + BK - 1) / BK * BK;
   // k-blocked staging: K tiles of BK over k_in become separate outer-K
   // blocks, and a K tile's operand rows are adjacent BK-element runs
@@ -518,14 +529,16 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   }
   const bool full = c.MD % BM == 0 && c.ND % BN == 0;
   const bool int32 = SP.total < (1LL << 31) && SQ.total < (1LL << 31);
-  // the pipelined tile addresses each operand with 32-bit byte offsets
-  const bool pipe = !knobs.igemm_plain && SP.total < (1LL << 30) && SQ.total < (1LL << 30);
+  // the pipelined tiles address each operand with 32-bit byte offsets
+  const bool pipe = (!knobs.igemm_plain || f32) && SP.total * EB < (1LL << 31) &&
+                    SQ.total * EB < (1LL << 31);
+  if (f32 && !pipe) return false;  // float operands: the pipelined tile only
   // pipelined tile: wave grid (4 waves as 2x2, or 8 as 2x4 / 4x2) and LDS rows
   int WGM = 2, WGN = 2;
   bool swz = false;
   if (pipe) {
     auto fits = [&](int wgm, int wgn) {
-      const int NT = 64 * wgm * wgn, CPR = BK / 8;
+      const int NT = 64 * wgm * wgn, CPR = BK * EB / 16;
       return BM % (32 * wgm) == 0 && BN % (32 * wgn) == 0 && BM * CPR % NT == 0 &&
              BN * CPR % NT == 0 && BM * CPR >= NT && BN * CPR >= NT;
     };
@@ -541,12 +554,12 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     }
     if (!fits(WGM, WGN)) WGM = WGN = 2;  // the tile is too small for that wave grid
     if (!fits(WGM, WGN)) return false;
-    swz = knobs.lds_swz == 1;
+    swz = knobs.lds_swz == 1 && !f32;
   }
   // halo form: Q rows of all taps of a K block loaded once per block
   int64_t HL = 0;
   int ntap = 0;
-  if (pipe && want_halo && blocked) HL = halo_rows(c, SQ, KO, BN, BK, ntap);
+  if (pipe && want_halo && blocked && !f32) HL = halo_rows(c, SQ, KO, BN, BK, ntap);
   if (HL) {
     // eight waves (2 x 4) where the tile allows, else four (2 x 2)
     auto fits = [&](int wgm, int wgn) {
@@ -559,8 +572,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     if (!fits(WGM, WGN) || 2 * (BM + HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
   }
   if (want_halo && !HL) return false;
-  const int lds = HL ? (int)(2 * (BM + HL) * (BK + 8) * 2)
-                     : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
+  const int lds = HL    ? (int)(2 * (BM + HL) * (BK + 8) * 2)
+                  : f32 ? 2 * (BM + BN) * (BK + 4) * 4
+                        : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;
 
   // ---- buffers: one staged (16-bit) scratch buffer per operand ----
@@ -569,7 +583,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   for (const auto &b : plan.buffers) scratch_no += b.kind == BufferSpec::SCRATCH;
   for (int o = 0; o < 2; ++o) {
     sbuf[o] = (int)plan.buffers.size();
-    int64_t words = ((st[o].total + 1) / 2 + 7) / 8 * 8;
+    int64_t words = ((st[o].total * EB + 3) / 4 + 7) / 8 * 8;
     auto nr = df.ir().inputs().at(c.ops[o].input_index);
     plan.buffers.push_back({BufferSpec::SCRATCH, scratch_no++, words, nr});
   }
@@ -600,7 +614,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     s.block[0] = 256;
     for (int o = 0; o < 2; ++o)
       s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
-                 st[o].total * 2;
+                 st[o].total * EB;
     plan.stages.push_back(s);
   }
   // geometry of the tile
@@ -649,7 +663,11 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   const std::string j0 =
       "(long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " + std::to_string(BN);
   const std::string ind = "  ";
-  if (HL)
+  if (f32)
+    k << ind << "lt_igemm_tile_f32<" << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, " << i0
+      << ", " << j0 << ", smem, " << SP.total * 4 << "u, " << SQ.total * 4 << "u);\n";
+  else if (HL)
     k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL
       << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2 << "u, "
@@ -667,7 +685,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
 
   Stage g;
   g.kernel = "lt_igemm0";
-  g.kind = bf16 ? "contract_bf16" : "contract_f16";
+  g.kind = bf16 ? "contract_bf16" : f32 ? "contract_f32" : "contract_f16";
   g.buffers = {sbuf[c.po], sbuf[c.qo], out_buf};
   g.writes = {out_buf};
   g.grid[0] = (unsigned)gx;
@@ -675,11 +693,11 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   g.grid[2] = (unsigned)c.BD;
   g.block[0] = (unsigned)threads;
   g.flops = 2 * c.MD * c.ND * c.KD * c.BD;
-  g.bytes = SP.total * 2 + SQ.total * 2 + c.MD * c.ND * c.BD * 4;
+  g.bytes = (SP.total + SQ.total) * EB + c.MD * c.ND * c.BD * 4;
   plan.stages.push_back(g);
   plan.source = k.str();
   std::stringstream desc;
-  desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : "f16")
+  desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : f32 ? "f32" : "f16")
        << " staged operands, f32 accumulate): D[" << c.BD << "][" << c.MD << "][" << c.ND
        << "] = P x Q over K=" << ko_n << "x" << (blocked ? KIT : KI) << (KIP != KI && !blocked ? " (padded to " + std::to_string(KIP) + ")" : std::string()) << ", tile " << BM << "x" << BN << "x"
        << BK << (pipe ? " pipelined, " + std::to_string(WGM * WGN) + " waves" +

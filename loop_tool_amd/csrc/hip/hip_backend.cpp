@@ -21,8 +21,11 @@
 //   vec=N          elements per lane for row reductions
 //   tile=MxN       workgroup tile of contractions
 //   bk=N           K tile (32 or 64) of 16-bit contractions
-//   contract=gather  16-bit contractions gather their operands in the tile
-//                  loader instead of staging them
+//   contract=gather  contractions gather their operands in the tile loader
+//                  instead of staging them
+//   contract=staged  float contractions are staged (and run the pipelined
+//                  float tile) whatever their size; by default only nests of
+//                  2^28 or more multiply-adds are
 //   igemm=plain    staged 16-bit contractions use the single-stage tile
 //                  (default: the software-pipelined tile where it applies)
 //   waves=2|4|8    waves per workgroup of the pipelined tile
@@ -177,6 +180,7 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       else if (tok == "mfma_bf16") k.bf16 = true;
       else if (tok == "mfma_f16") k.f16 = true;
       else if (tok == "contract=gather") k.gather = true;
+      else if (tok == "contract=staged") k.staged = true;
       else if (tok == "igemm=plain") k.igemm_plain = true;
       else if (tok == "igemm=halo") k.igemm_halo = 1;
       else if (tok == "igemm=pipe") k.igemm_halo = 0;

@@ -534,4 +534,138 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
 }
 
+// FP32 form of the pipelined tile (v_mfma_f32_32x32x2_f32) for staged float
+// operands, KI % BK == 0.  Same pipeline as lt_igemm_tile_pipe: global loads
+// two K tiles ahead, double-buffered LDS rows padded by 16 bytes, one barrier
+// per K tile.  Per k16 step each lane reads 8 consecutive k of its fragment
+// row (two 16-byte LDS reads): lanes 0-31 hold k 0..7 and lanes 32-63 k 8..15
+// of the step, so MFMA m (m = 0..7) adds the products of k = m and k = 8 + m.
+template <int BM, int BN, int BK>
+__host__ __device__ constexpr int lt_igemm_f32_lds_bytes() {
+  return 2 * (BM + BN) * (BK + 4) * 4;
+}
+
+template <int BM, int BN, int BK, bool FULL, int WGM, int WGN, class G>
+__device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __restrict__ P,
+                                                  const float* __restrict__ Q,
+                                                  float* __restrict__ out, long i0, long j0,
+                                                  unsigned char* smem, unsigned pbytes,
+                                                  unsigned qbytes) {
+  constexpr int NT = 64 * WGM * WGN, LDR = BK + 4;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+  constexpr int CPR = BK / 4, KK = BK / 16;   // 16-byte chunks per row, k16 steps
+  constexpr int VP = BM * CPR / NT, VQ = BN * CPR / NT;
+  static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
+  static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
+                "tile too small for the workgroup");
+  static_assert(KK % 2 == 0, "BK must be a multiple of 32");
+  const int KIT = g.KI / BK;
+  const int KT = g.KO * KIT;
+  float* sP = (float*)smem;      // [2][BM][LDR]
+  float* sQ = sP + 2 * BM * LDR; // [2][BN][LDR]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const int cc = (tid % CPR) * 4;
+  const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
+  const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
+  unsigned pv[VP], qv[VQ];
+#pragma unroll
+  for (int v = 0; v < VP; ++v) {
+    bool ok;
+    const long o = g.p_row(i0 + (tid + v * NT) / CPR, ok);
+    pv[v] = (FULL || ok) ? (unsigned)((o + cc) * 4) : 0x80000000u;
+  }
+#pragma unroll
+  for (int v = 0; v < VQ; ++v) {
+    bool ok;
+    const long o = g.q_row(j0 + (tid + v * NT) / CPR, ok);
+    qv[v] = (FULL || ok) ? (unsigned)((o + cc) * 4) : 0x80000000u;
+  }
+  lt_ig_u32x4 r0p[VP], r0q[VQ], r1p[VP], r1q[VQ];
+  auto load = [&](int kt, lt_ig_u32x4* rp, lt_ig_u32x4* rq) {
+    const int ko = kt / KIT, kb = (kt % KIT) * BK;
+    const int po = (int)((g.p_koff(ko) + kb) * 4), qo = (int)((g.q_koff(ko) + kb) * 4);
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      rp[v] = __builtin_amdgcn_raw_buffer_load_b128(rsP, (int)pv[v], po, 0);
+#pragma unroll
+    for (int v = 0; v < VQ; ++v)
+      rq[v] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)qv[v], qo, 0);
+  };
+  auto store = [&](int buf, const lt_ig_u32x4* rp, const lt_ig_u32x4* rq) {
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      *(lt_ig_u32x4*)(sP + (buf * BM + (tid + v * NT) / CPR) * LDR + cc) = rp[v];
+#pragma unroll
+    for (int v = 0; v < VQ; ++v)
+      *(lt_ig_u32x4*)(sQ + (buf * BN + (tid + v * NT) / CPR) * LDR + cc) = rq[v];
+  };
+  typedef float F4 __attribute__((ext_vector_type(4)));
+  F4 fa[2][TM][2], fb[2][TN][2];
+  const int fr = lane & 31, fh = lane >> 5;
+  auto frags = [&](int buf, int kk, int s) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a) {
+      const float* p = sP + (buf * BM + wm * WM + a * 32 + fr) * LDR + kk * 16 + fh * 8;
+      fa[s][a][0] = *(const F4*)p;
+      fa[s][a][1] = *(const F4*)(p + 4);
+    }
+#pragma unroll
+    for (int b = 0; b < TN; ++b) {
+      const float* p = sQ + (buf * BN + wn * WN + b * 32 + fr) * LDR + kk * 16 + fh * 8;
+      fb[s][b][0] = *(const F4*)p;
+      fb[s][b][1] = *(const F4*)(p + 4);
+    }
+  };
+  lt_ig_f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  auto step = [&](int kt, int B, const lt_ig_u32x4* RNp, const lt_ig_u32x4* RNq,
+                  lt_ig_u32x4* RFp, lt_ig_u32x4* RFq, bool ld, bool st) {
+    if (ld) load(kt + 2, RFp, RFq);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const int s = kk & 1;
+      if (kk + 1 < KK) frags(B, kk + 1, s ^ 1);
+      if (kk == KK - 1 && st) store(B ^ 1, RNp, RNq);
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int a = 0; a < TM; ++a)
+#pragma unroll
+          for (int b = 0; b < TN; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                fa[s][a][m >> 2][m & 3], fb[s][b][m >> 2][m & 3], acc[a][b], 0, 0, 0);
+    }
+    if (st) {
+      __syncthreads();
+      frags(B ^ 1, 0, 0);
+    }
+  };
+  load(0, r0p, r0q);
+  if (KT > 1) load(1, r1p, r1q);
+  store(0, r0p, r0q);
+  __syncthreads();
+  frags(0, 0, 0);
+  int kt = 0;
+  for (; kt + 3 < KT; kt += 2) {
+    step(kt, 0, r1p, r1q, r0p, r0q, true, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, true, true);
+  }
+  const int R = KT - kt;
+  if (R == 3) {
+    step(kt, 0, r1p, r1q, r0p, r0q, true, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, false, true);
+    step(kt + 2, 0, r1p, r1q, r0p, r0q, false, false);
+  } else if (R == 2) {
+    step(kt, 0, r1p, r1q, r0p, r0q, false, true);
+    step(kt + 1, 1, r0p, r0q, r1p, r1q, false, false);
+  } else {
+    step(kt, 0, r1p, r1q, r0p, r0q, false, false);
+  }
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+}
+
 #endif  // LT_IGEMM_TILE_H_

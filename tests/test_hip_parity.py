@@ -120,6 +120,26 @@ def test_contraction_f32_vs_oracle(lt, hip_dev, name, kw):
 
 
 @pytest.mark.parametrize("name,kw", CONTRACT, ids=["%s%d" % (c[0], i) for i, c in enumerate(CONTRACT)])
+@pytest.mark.parametrize("knobs", ["contract=staged", "contract=staged tile=128x128",
+                                   "contract=staged tile=64x128 waves=8",
+                                   "contract=staged stage=flat"])
+def test_contraction_f32_staged_vs_oracle(lt, hip_dev, name, kw, knobs):
+    """FP32 contraction on staged float operands (one staging launch, then
+    the pipelined float tile on v_mfma_f32_32x32x2_f32): <= 1e-5 normwise vs
+    the C oracle and vs float64."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    y_ref, _ = run_backend(lt, out, shapes, inputs, "cpp")
+    y_hip, cc = run_backend(lt, out, shapes, inputs, "hip", annotate=knobs)
+    assert cc.string_property("kinds") == "stage,contract_f32"
+    f64 = P.reference_output(name, inputs, **ref_kw(name, kw)).ravel()
+    print(name, kw, knobs, "hip-vs-oracle %.3g hip-vs-f64 %.3g" % (
+        normwise(y_hip, y_ref), normwise(y_hip, f64)))
+    assert normwise(y_hip, y_ref) <= 1e-5
+    assert normwise(y_hip, f64) <= 1e-5
+
+
+@pytest.mark.parametrize("name,kw", CONTRACT, ids=["%s%d" % (c[0], i) for i, c in enumerate(CONTRACT)])
 def test_contraction_bf16_vs_f64(lt, hip_dev, name, kw):
     """BF16 operands on MFMA with FP32 accumulation: <= 1e-2 normwise vs a
     float64 evaluation, and exactly the float64 result of the bf16-rounded

@@ -41,7 +41,7 @@ static int g_phase;
 """
 
 DRIVER = r"""
-typedef void (*stage_fn)(const float*, __bf16*, unsigned);
+typedef void (*stage_fn)(const float*, LT_ELT*, unsigned);
 static void* slurp(const char* path, size_t n) {
   void* p = malloc(n);
   FILE* fp = fopen(path, "rb");
@@ -57,14 +57,14 @@ int main(int argc, char** argv) {
     char path[4096];
     snprintf(path, sizeof path, "%s/in%d.bin", argv[1], o);
     float* in = (float*)slurp(path, n * 4);
-    unsigned short* S = (unsigned short*)malloc((s + 64) * 2);
-    memset(S, 0xA5, (s + 64) * 2);
+    unsigned char* S = (unsigned char*)malloc((s + 64) * sizeof(LT_ELT));
+    memset(S, 0xA5, (s + 64) * sizeof(LT_ELT));
     for (unsigned b = 0; b < lt_stage_blocks[o]; ++b)
       for (g_phase = 0; g_phase < 2; ++g_phase)
-        for (threadIdx.x = 0; threadIdx.x < 256; ++threadIdx.x) fn[o](in, (__bf16*)S, b);
+        for (threadIdx.x = 0; threadIdx.x < 256; ++threadIdx.x) fn[o](in, (LT_ELT*)S, b);
     snprintf(path, sizeof path, "%s/S%d.bin", argv[1], o);
     FILE* fp = fopen(path, "wb");
-    fwrite(S, 2, s + 64, fp);
+    fwrite(S, sizeof(LT_ELT), s + 64, fp);
     fclose(fp);
     free(in);
     free(S);
@@ -90,35 +90,45 @@ def extract(src, name):
             return src[i:j]
 
 
-def staged_conv_input(X, pad, k, stride, kip):
-    """X[N][C][H][W] -> X'[N][H+lo+hi][W+lo+hi][kip] (bf16 bits)."""
+def f32_bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def staged_conv_input(X, pad, k, stride, kip, bits=None):
+    """X[N][C][H][W] -> X'[N][H+lo+hi][W+lo+hi][kip] (bf16 bits, or `bits`)."""
     N, C, H, W = X.shape
     Ho, Wo = (H + 2 * pad - k) // stride + 1, (W + 2 * pad - k) // stride + 1
     hi_h = max(0, (Ho - 1) * stride + k - 1 - pad - (H - 1))
     hi_w = max(0, (Wo - 1) * stride + k - 1 - pad - (W - 1))
-    out = np.zeros((N, H + pad + hi_h, W + pad + hi_w, kip), np.uint16)
-    out[:, pad:pad + H, pad:pad + W, :C] = bf16_bits(X.transpose(0, 2, 3, 1))
+    bits = bits or bf16_bits
+    v = bits(X.transpose(0, 2, 3, 1))
+    out = np.zeros((N, H + pad + hi_h, W + pad + hi_w, kip), v.dtype)
+    out[:, pad:pad + H, pad:pad + W, :C] = v
     return out
 
 
-def staged_weights(Wt, kip):
-    """W[Co][C][k][k] -> W'[Co][k][k][kip] (bf16 bits)."""
+def staged_weights(Wt, kip, bits=None):
+    """W[Co][C][k][k] -> W'[Co][k][k][kip] (bf16 bits, or `bits`)."""
     Co, C, k, _ = Wt.shape
-    out = np.zeros((Co, k, k, kip), np.uint16)
-    out[..., :C] = bf16_bits(Wt.transpose(0, 2, 3, 1))
+    bits = bits or bf16_bits
+    v = bits(Wt.transpose(0, 2, 3, 1))
+    out = np.zeros((Co, k, k, kip), v.dtype)
+    out[..., :C] = v
     return out
 
 
-def run_staging(lt, out, inputs, tmp, annotate=""):
+def run_staging(lt, out, inputs, tmp, annotate="", backend="hip_bf16"):
     t = out.loop_tree
     if annotate:
         t = t.annotate(t.roots[0], annotate)
-    cc = lt.compile(t, "hip_bf16")
-    assert cc.string_property("kinds") == "stage,contract_bf16"
+    cc = lt.compile(t, backend)
+    kind = "contract_bf16" if backend == "hip_bf16" else "contract_f32"
+    assert cc.string_property("kinds") == "stage," + kind
     src = cc.string_property("code")
     blocks = re.search(r"static constexpr unsigned lt_stage_blocks\[2\] = \{[^}]*\};", src).group(0)
-    body = (PRELUDE + blocks + "\n" + extract(src, "lt_stage_op0") + "\n" +
-            extract(src, "lt_stage_op1") + "\n" + DRIVER)
+    elt = "__bf16" if backend == "hip_bf16" else "float"
+    body = (PRELUDE + "#define LT_ELT " + elt + "\n" + blocks + "\n" +
+            extract(src, "lt_stage_op0") + "\n" + extract(src, "lt_stage_op1") + "\n" + DRIVER)
     with open(os.path.join(tmp, "s.cpp"), "w") as f:
         f.write(body)
     # AddressSanitizer (host build): a source read outside the input is an error
@@ -128,13 +138,17 @@ def run_staging(lt, out, inputs, tmp, annotate=""):
 
 
 def check_staged(tmp, exp, numels):
+    """exp: expected staged operands as uint16 (16-bit staging) or uint32
+    (float staging) bit patterns."""
     args = [os.path.join(tmp, "s"), tmp]
     for o in range(2):
         args += [str(numels[o]), str(exp[o].size)]
     subprocess.run(args, check=True, timeout=300)
     for o in range(2):
-        S = np.fromfile(os.path.join(tmp, "S%d.bin" % o), np.uint16)
-        assert np.all(S[exp[o].size:] == 0xA5A5), "operand %d: write past the staged buffer" % o
+        dt = exp[o].dtype
+        S = np.fromfile(os.path.join(tmp, "S%d.bin" % o), dt)
+        guard = dt.type(0xA5A5A5A5 if dt == np.uint32 else 0xA5A5)
+        assert np.all(S[exp[o].size:] == guard), "operand %d: write past the staged buffer" % o
         bad = np.flatnonzero(S[:exp[o].size] != exp[o].ravel())
         assert bad.size == 0, "operand %d: %d wrong elements, first at %s of %s" % (
             o, bad.size, np.unravel_index(bad[0], exp[o].shape), exp[o].shape)
@@ -240,3 +254,40 @@ def test_matmul_blocked_staging_on_host(lt, M, N, K, tmpdir_):
     A.tofile(os.path.join(tmpdir_, "in0.bin"))
     B.tofile(os.path.join(tmpdir_, "in1.bin"))
     check_staged(tmpdir_, [blocked(ea, 32, (0,)), blocked(eb, 32, (0,))], [A.size, B.size])
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("kw", CONV, ids=[str(i) for i in range(len(CONV))])
+def test_conv_float_staging_on_host(lt, kw, tmpdir_):
+    """Float operands ("hip" backend, contract=staged): the same k-blocked
+    layouts as 16-bit staging, with the values copied exactly."""
+    pad, k, stride = kw.get("pad", 1), kw.get("k", 3), kw.get("stride", 1)
+    out, shapes = P.conv2d(lt, **kw)
+    X, Wt = P.uniform(7, shapes[0]), P.uniform(8, shapes[1])
+    cc = run_staging(lt, out, [X, Wt], tmpdir_, "contract=staged", backend="hip")
+    kip = -(-kw["Ci"] // 32) * 32
+    ex = staged_conv_input(X, pad, k, stride, kip, f32_bits)
+    ew = staged_weights(Wt, kip, f32_bits)
+    if kip > 32:
+        ex, ew = blocked(ex, 32, (0, 1, 2)), blocked(ew, 32, (1, 2, 0))
+    X.tofile(os.path.join(tmpdir_, "in0.bin"))
+    Wt.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, [ex, ew], [X.size, Wt.size])
+
+
+@pytest.mark.skipif(not os.path.exists(CLANG), reason="needs ROCm clang for the host build")
+@pytest.mark.parametrize("M,N,K", [(64, 96, 128), (37, 53, 29), (300, 200, 704)])
+def test_matmul_float_staging_on_host(lt, M, N, K, tmpdir_):
+    out, shapes = P.matmul(lt, M, N, K)
+    A, B = P.uniform(9, shapes[0]), P.uniform(10, shapes[1])
+    run_staging(lt, out, [A, B], tmpdir_, "contract=staged", backend="hip")
+    kip = -(-K // 32) * 32
+    ea = np.zeros((M, kip), np.uint32)
+    ea[:, :K] = f32_bits(A)
+    eb = np.zeros((N, kip), np.uint32)
+    eb[:, :K] = f32_bits(B.T)
+    if kip > 32:
+        ea, eb = blocked(ea, 32, (0,)), blocked(eb, 32, (0,))
+    A.tofile(os.path.join(tmpdir_, "in0.bin"))
+    B.tofile(os.path.join(tmpdir_, "in1.bin"))
+    check_staged(tmpdir_, [ea, eb], [A.size, B.size])
