@@ -59,8 +59,11 @@ struct Knobs {
   int stage_blocked = -1;      // "stage=blocked|flat": k-blocked operand staging
                                // (-1 = auto: blocked when k_in spans several K tiles)
   int kb_outer = -1;           // "kb=outer|inner": order of the K blocks
-  int igemm_halo = -1;         // "igemm=halo": staged 16-bit contractions load
-                               // the Q rows of all taps of a K block once
+  int igemm_halo = -1;         // "igemm=halo" (1) / "igemm=halo1" (2, one halo
+                               // buffer) / "igemm=pipe" (0): staged 16-bit
+                               // contractions load the Q rows of all taps of a
+                               // K block once (-1 = auto: one buffer where it
+                               // applies)
 };
 
 // Builds the whole launch plan for a LoopTree.

@@ -509,9 +509,12 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   const int64_t KIP = (KI + BK - 1) / BK * BK;
   // k-blocked staging: K tiles of BK over k_in become separate outer-K
   // blocks, and a K tile's operand rows are adjacent BK-element runs
-  const bool want_halo = knobs.igemm_halo == 1;
-  const bool blocked = (knobs.stage_blocked != 0 || want_halo) && KIP > BK;
-  const bool kb_first = knobs.kb_outer == 1 || want_halo;
+  // halo form: forced by igemm=halo|halo1, tried by default (one buffer)
+  // for 16-bit operands, off with igemm=pipe
+  const bool halo_forced = knobs.igemm_halo >= 1;
+  const bool want_halo = !f32 && knobs.igemm_halo != 0;
+  const bool blocked = (knobs.stage_blocked != 0 || halo_forced) && KIP > BK;
+  const bool kb_first = knobs.kb_outer == 1 || (want_halo && knobs.kb_outer != 0);
   Staged st[2] = {stage_layout(c.ops[0], dims[0], KIP, blocked ? BK : 0, c.K),
                   stage_layout(c.ops[1], dims[1], KIP, blocked ? BK : 0, c.K)};
   const Staged &SP = st[c.po], &SQ = st[c.qo];
@@ -559,7 +562,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   // halo form: Q rows of all taps of a K block loaded once per block
   int64_t HL = 0;
   int ntap = 0;
-  if (pipe && want_halo && blocked && !f32) HL = halo_rows(c, SQ, KO, BN, BK, ntap);
+  const int HB = knobs.igemm_halo == 1 ? 2 : 1;  // halo buffers
+  if (pipe && want_halo && blocked && kb_first) HL = halo_rows(c, SQ, KO, BN, BK, ntap);
   if (HL) {
     // eight waves (2 x 4) where the tile allows, else four (2 x 2)
     auto fits = [&](int wgm, int wgn) {
@@ -569,10 +573,10 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     };
     WGM = 2;
     WGN = knobs.waves != 4 && fits(2, 4) ? 4 : 2;
-    if (!fits(WGM, WGN) || 2 * (BM + HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
+    if (!fits(WGM, WGN) || (2 * BM + HB * HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
   }
-  if (want_halo && !HL) return false;
-  const int lds = HL    ? (int)(2 * (BM + HL) * (BK + 8) * 2)
+  if (halo_forced && !HL) return false;
+  const int lds = HL    ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
                         : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;
@@ -669,7 +673,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
       << ", " << j0 << ", smem, " << SP.total * 4 << "u, " << SQ.total * 4 << "u);\n";
   else if (HL)
     k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
-      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL << ", " << HB
       << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2 << "u, "
       << SQ.total * 2 << "u);\n";
   else if (pipe)
@@ -705,7 +709,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
                       : std::string())
        << (blocked ? std::string(", k-blocked staging (kb ") + (kb_first ? "outer)" : "inner)")
                    : std::string())
-       << (HL ? ", Q halo of " + std::to_string(HL) + " rows per K block" : std::string())
+       << (HL ? ", Q halo of " + std::to_string(HL) + " rows per K block" +
+                    (HB == 1 ? " (one buffer)" : "")
+              : std::string())
        << "; staged P " << SP.total << " and Q " << SQ.total
        << " elements\n";
   plan.description = desc.str();

@@ -34,9 +34,11 @@
 //   stage=blocked|flat  16-bit contraction operands staged k-blocked
 //                  ([K/BK][...][BK]) or with the whole k_in dim innermost
 //   kb=outer|inner order of the K blocks among the outer-K positions
-//   igemm=halo|pipe  staged 16-bit contractions whose outer-K positions shift
-//                  the Q rows (convolution taps) load those rows once per
-//                  K block into an LDS halo (halo) or per K tile (pipe)
+//   igemm=halo|halo1|pipe  staged 16-bit contractions whose outer-K positions
+//                  shift the Q rows (convolution taps) load those rows once
+//                  per K block into an LDS halo, double-buffered (halo) or
+//                  single (halo1, the default where it applies), or per K
+//                  tile (pipe)
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -183,6 +185,7 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       else if (tok == "contract=staged") k.staged = true;
       else if (tok == "igemm=plain") k.igemm_plain = true;
       else if (tok == "igemm=halo") k.igemm_halo = 1;
+      else if (tok == "igemm=halo1") k.igemm_halo = 2;
       else if (tok == "igemm=pipe") k.igemm_halo = 0;
       else if (tok == "lds=swz") k.lds_swz = 1;
       else if (tok == "lds=pad") k.lds_swz = 0;

@@ -372,14 +372,17 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
 // position-major staged input X'[C/BK][N][H+2][W+2][BK]).  K tiles run block
 // by block, taps inner.  The Q rows all taps of one block need are the HL
 // consecutive BK-vectors starting at the tile's first row: they are loaded
-// once per block into an LDS halo buffer (double-buffered; the next block's
-// halo is loaded during the first tap and stored to LDS after the last) and
+// once per block into an LDS halo buffer (HB = 2: double-buffered, the next
+// block's halo loaded during the first tap and stored to LDS after the last;
+// HB = 1: one buffer, rewritten between two barriers after the last tap, for
+// half the LDS and two workgroups per CU) and
 // every tap reads its fragments from it at a uniform shift, instead of
 // reloading the tile's rows from L2 for every tap.  P goes through the same
 // two-tiles-ahead register pipeline and double-buffered LDS as in
 // lt_igemm_tile_pipe.  HL (a host-computed bound over all tiles) must cover
 // (q_row(j_last) - q_row(j0) + max tap shift) / BK + 1.
-template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, class G>
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, int HB,
+          class G>
 __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restrict__ P,
                                                    const T* __restrict__ Q,
                                                    float* __restrict__ out, long i0, long j0,
@@ -398,7 +401,8 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   static_assert(NTAP >= 2, "the halo form needs at least two taps per K block");
   const int KT = g.KO, KB = KT / NTAP;
   T* sP = (T*)smem;              // [2][BM][LDR]
-  T* sH = sP + 2 * BM * LDR;     // [2][HL][LDR]
+  T* sH = sP + 2 * BM * LDR;     // [HB][HL][LDR]
+  static_assert(HB == 1 || HB == 2, "one or two halo buffers");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
   const int ck = tid % CPR, cc = ck * 8;
@@ -465,7 +469,8 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   // LDS element offset of tile kt's tap shift within its block's halo buffer
   auto hoff = [&](int kt) {
     const int kb = kt / NTAP;
-    return (kb & 1) * HL * LDR + (int)((g.q_koff(kt) - g.q_koff(kb * NTAP)) / BK) * LDR;
+    return (HB == 2 ? (kb & 1) * HL * LDR : 0) +
+           (int)((g.q_koff(kt) - g.q_koff(kb * NTAP)) / BK) * LDR;
   };
   V fa[2][TM], fb[2][TN];
   auto frags = [&](int buf, int ho, int kk, int s) {
@@ -501,9 +506,14 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
     }
-    if (tap == NTAP - 1 && next_halo) hstore((kb + 1) & 1);
+    if (HB == 2 && tap == NTAP - 1 && next_halo) hstore((kb + 1) & 1);
     if (st) {
       __syncthreads();
+      if (HB == 1 && tap == NTAP - 1 && next_halo) {
+        // one halo buffer: every wave is past block kb's last reads
+        hstore(0);
+        __syncthreads();
+      }
       frags(B ^ 1, hoff(kt + 1), 0, 0);
     }
   };

@@ -212,7 +212,8 @@ HALO = [
 @pytest.mark.parametrize("tile", ["", "tile=128x128", "tile=64x128", "tile=128x256 waves=4"])
 def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     """The halo form of the staged tile (the Q rows of all window taps of a
-    channel block loaded once into LDS) computes the same K order as the
+    channel block loaded once into LDS, in two buffers or one) computes the
+    same K order as the
     pipelined tile over the same k-blocked staging (blocks outer, taps
     inner): the two results must be bit-identical, and within float32
     accumulation error of float64 over the bf16-rounded operands.  Covers
@@ -221,16 +222,17 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     ys = []
-    for knobs in ("igemm=halo", "stage=blocked kb=outer"):
+    for knobs in ("igemm=pipe stage=blocked kb=outer", "igemm=halo", "igemm=halo1"):
         t = out.loop_tree
         t = t.annotate(t.roots[0], (knobs + " " + tile).strip())
         cc = lt.compile(t, "hip_bf16")
         assert cc.string_property("kinds") == "stage,contract_bf16"
-        assert ("Q halo" in cc.dump()) == (knobs == "igemm=halo")
+        assert ("Q halo" in cc.dump().splitlines()[0]) == knobs.startswith("igemm=halo")
         y = np.zeros(t.io_numel()[len(inputs)], np.float32)
         cc.run(inputs + [y])
         ys.append(y)
-    assert np.array_equal(ys[0].view(np.uint32), ys[1].view(np.uint32))
+    for y in ys[1:]:
+        assert np.array_equal(y.view(np.uint32), ys[0].view(np.uint32))
 
     def bf16(a):
         u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
