@@ -4,7 +4,8 @@
 //   lt_ggemm_<t>(descs, items, n_items)              persistent grouped launch
 //   lt_cvt_<t>(src, dst, rows, cols, transpose)      float32 -> 16-bit operands
 //
-// Both launch forms run the implicit-GEMM tile of igemm_tile.h on operands
+// Both launch forms run the software-pipelined implicit-GEMM tile of
+// igemm_tile.h (with its K-tail masking: K is any multiple of 8) on operands
 // A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
 // kernel walks a host-built work list of output tiles from many GEMMs
 // (ordered by decreasing cost); workgroup w handles items w, w + grid,
@@ -64,13 +65,15 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
       lt_gemm_##TN##_##BM##x##BN##x##BK(const T* __restrict__ A, const T* __restrict__ Bt,  \
                                         float* __restrict__ C, int M, int N, int K) {       \
     __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<BM, BN, BK>()]; \
-    lt_igemm_tile<T, BM, BN, BK>(lt_mm_geo(M, N, K), A, Bt, C, (long)blockIdx.y * BM,     \
-                                 (long)blockIdx.x * BN, smem);                             \
+    lt_igemm_tile_pipe<T, BM, BN, BK, false, 2, 2, false, true>(                            \
+        lt_mm_geo(M, N, K), A, Bt, C, (long)blockIdx.y * BM, (long)blockIdx.x * BN, smem,  \
+        (unsigned)M * K * 2, (unsigned)N * K * 2);                                          \
   }
 
 #define LT_CASE(i, BM, BN, BK)                                                               \
   case i:                                                                                    \
-    lt_igemm_tile<T, BM, BN, BK>(g, A, Bt, d.C, it.m0, it.n0, smem);                        \
+    lt_igemm_tile_pipe<T, BM, BN, BK, false, 2, 2, false, true>(                            \
+        g, A, Bt, d.C, it.m0, it.n0, smem, (unsigned)d.M * d.K * 2, (unsigned)d.N * d.K * 2); \
     break;
 
 #define LT_GROUPED(T_, TN)                                                                   \

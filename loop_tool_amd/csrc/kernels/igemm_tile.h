@@ -187,7 +187,9 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 }
 
 // Software-pipelined form of the same tile, for staged operands of fewer than
-// 2^31 bytes with KI % BK == 0 (pbytes, qbytes: operand sizes in bytes).
+// 2^31 bytes (pbytes, qbytes: operand sizes in bytes) with KI % BK == 0, or,
+// with KTAIL, KI % 8 == 0: the last K tile then zeroes the 16-byte chunks
+// past KI (they hold the next row's values, or the buffer range check's 0).
 //  * WGM x WGN waves (4 or 8) split the tile; each wave owns
 //    (BM/WGM) x (BN/WGN) as 32x32 MFMA blocks.  Eight waves put two waves on
 //    every SIMD, so one wave's MFMAs cover the other's LDS and load waits.
@@ -224,7 +226,8 @@ __device__ __forceinline__ int lt_ig_lds_off(int row, int chunk) {
   return row * (BK + 8) + (chunk << 3);
 }
 
-template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool SWZ, class G>
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool SWZ,
+          bool KTAIL = false, class G>
 __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restrict__ P,
                                                    const T* __restrict__ Q,
                                                    float* __restrict__ out, long i0, long j0,
@@ -239,7 +242,7 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
   static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
                 "tile too small for the workgroup");
   static_assert(KK % 2 == 0, "BK must be a multiple of 32");
-  const int KIT = g.KI / BK;
+  const int KIT = KTAIL ? (g.KI + BK - 1) / BK : g.KI / BK;
   const int KT = g.KO * KIT;
   T* sP = (T*)smem;              // [2][BM][LDR]
   T* sQ = sP + 2 * BM * LDR;     // [2][BN][LDR]
@@ -278,6 +281,12 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
 #pragma unroll
     for (int v = 0; v < VQ; ++v)
       rq[v] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)qv[v], qo, 0);
+    if (KTAIL && kb + cc >= g.KI) {  // this lane's chunk lies past the K extent
+#pragma unroll
+      for (int v = 0; v < VP; ++v) rp[v] = (lt_ig_u32x4){};
+#pragma unroll
+      for (int v = 0; v < VQ; ++v) rq[v] = (lt_ig_u32x4){};
+    }
   };
   auto store = [&](int buf, const lt_ig_u32x4* rp, const lt_ig_u32x4* rq) {
 #pragma unroll
