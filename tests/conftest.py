@@ -30,13 +30,22 @@ def lt():
 
 
 @pytest.fixture(scope="session")
-def ref():
+def ref_opt():
+    """The reference library built from its sources (oracle/_ref), or None
+    where it is not built (no /root/reference)."""
     if not os.path.exists(REF_LIB):
-        pytest.skip("reference library oracle/_ref not built")
+        return None
     from loop_tool_amd import capi
     # the reference's C backend compiles to fixed /tmp names: one compile at a
     # time across the test processes
     return capi.Library(REF_LIB, compile_lock="/tmp/lt_ref_cpp_backend.lock")
+
+
+@pytest.fixture(scope="session")
+def ref(ref_opt):
+    if ref_opt is None:
+        pytest.skip("reference library oracle/_ref not built")
+    return ref_opt
 
 
 @pytest.fixture(scope="session")
