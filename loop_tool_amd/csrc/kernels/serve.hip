@@ -207,6 +207,7 @@ struct LtSvGeo {
   long qd1, qd2, qc0, qc1, qc2, qcst;
   long ko2, pk0, pk1, qk0, qk1;
   long o_rs, oc0, oc1, oc2;
+  unsigned pbytes, qbytes;  // staged operand sizes (the tile's buffer range)
   __device__ __forceinline__ long p_row(long i, bool& ok) const {
     ok = i < MD;
     return i * p_rs;
@@ -230,8 +231,9 @@ struct LtSvGeo {
       lt_sv_igemm_##TN##_##BM##x##BN##_##BK(LtSvGeo g, const T* __restrict__ P,             \
                                             const T* __restrict__ Q, float* __restrict__ out) { \
     __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<BM, BN, BK>()]; \
-    lt_igemm_tile<T, BM, BN, BK>(g, P, Q, out, (long)blockIdx.y * BM,                      \
-                                 (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * BN, smem); \
+    lt_igemm_tile_pipe<T, BM, BN, BK, false, 2, 2, false>(                                  \
+        g, P, Q, out, (long)blockIdx.y * BM, (long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * BN, \
+        smem, g.pbytes, g.qbytes);                                                          \
   }
 #define LT_SV_IGEMM_ALL(T, TN)      \
   LT_SV_IGEMM(T, TN, 64, 64, 32)    \

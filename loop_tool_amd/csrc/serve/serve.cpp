@@ -108,6 +108,7 @@ struct Geo {
   int32_t KO, KI;
   int64_t MD, ND, p_rs, qd1, qd2, qc0, qc1, qc2, qcst, ko2, pk0, pk1, qk0, qk1, o_rs, oc0,
       oc1, oc2;
+  uint32_t pbytes, qbytes;  // staged operand sizes (the tile's buffer range)
 };
 
 void launch(void *fn, int64_t blocks, unsigned block, void *stream, std::vector<void *> args,
@@ -297,6 +298,8 @@ bool plan_matmul(const Structure &s, int dtype, Plan &p) {
   p.in_numel = {M * K, K * N};
   p.out_numel = M * N;
   p.scratch_bytes = {M * KP * 2, N * KP * 2};
+  // the tile addresses staged operands with 32-bit byte offsets
+  if (p.scratch_bytes[0] >= (1LL << 31) || p.scratch_bytes[1] >= (1LL << 31)) return false;
   // A[M][K] -> A'[M][KP]; B[K][N] -> Bt'[N][KP] (lanes along n, contiguous in B)
   p.st[0] = stage({1, 1, M, KP}, {0, 0, K, 1}, {1, 1, M, K}, {0, 0, 0, 0}, K, -1);
   p.st[1] = stage({1, 1, N, KP}, {0, 0, 1, N}, {1, 1, N, K}, {0, 0, 0, 0}, K, 2);
@@ -330,6 +333,8 @@ bool plan_conv(const Structure &s, int dtype, Plan &p) {
   p.in_numel = {N * C * H * W, Co * C * 9};
   p.out_numel = N * Co * H * W;
   p.scratch_bytes = {N * Hp * Wp * CP * 2, Co * 9 * CP * 2};
+  // the tile addresses staged operands with 32-bit byte offsets
+  if (p.scratch_bytes[0] >= (1LL << 31) || p.scratch_bytes[1] >= (1LL << 31)) return false;
   // X[N][C][H][W] -> X'[N][H+2][W+2][CP] (zero halo); W[Co][C][3][3] -> W'[Co][3][3][CP]
   p.st[0] = stage({N, Hp, Wp, CP}, {C * H * W, W, 1, H * W}, {N, H, W, C}, {0, 1, 1, 0}, C, 2);
   p.st[1] = stage({Co, 3, 3, CP}, {C * 9, 3, 1, 9}, {Co, 3, 3, C}, {0, 0, 0, 0}, C, 2);
@@ -571,7 +576,9 @@ struct Server {
       return p;
     }
     // any other program (or sizes a family kernel does not take): the hip
-    // backend, compiled once per exact lazy hash
+    // backend, compiled once per exact lazy hash; a family plan that gave up
+    // may have filled fields
+    *p = Plan();
     p->family = LTS_GENERIC;
     uint64_t h = t.hash();
     auto it = compiled.find(h);
@@ -657,6 +664,8 @@ struct Server {
         launch(K.fn(p.stage_fn), grid_for(nvec, 256, 16384), 256, s, {&d, &src, &dst});
       }
       Geo g = p.geo;
+      g.pbytes = (uint32_t)p.scratch_bytes[p.p_of];
+      g.qbytes = (uint32_t)p.scratch_bytes[1 - p.p_of];
       const void *P = scratch[p.p_of], *Q = scratch[1 - p.p_of];
       void *out = r.out;
       launch(K.fn(p.gemm_fn), p.gx, 256, s, {&g, &P, &Q, &out}, p.gy, 1);
