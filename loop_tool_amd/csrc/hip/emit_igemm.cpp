@@ -1,18 +1,24 @@
-// Staged-operand contraction lowering for 16-bit MFMA operands.
+// Staged-operand contraction lowering: 16-bit MFMA operands (bf16, fp16)
+// with fp32 accumulation, and float operands for large nests.
 //
 // For a contraction nest (see emit_contract.cpp for the analysis) whose two
 // operands share a reduced var k_in that indexes one dim of each operand
 // alone (coefficient 1, no offset), the plan is
-//   stage 0     one launch that stages both operands: permute each so
-//               k_in's dim is innermost, pad every dim that a view can index
-//               out of range with a zero halo, zero-pad k_in to a multiple
-//               of the K tile, round to bf16/fp16 (RNE);
-//   stage 1     the implicit-GEMM tile of csrc/kernels/igemm_tile.h: every
-//               operand access is now row_base(row) + outer_offset(k_out) +
-//               k_in with no bounds checks, loaded as 16-byte vectors.
-// For a 3x3 pad-1 convolution X[N][C][H][W] * W[Co][C][3][3] this stages
-// X'[N][H+2][W+2][C] and W'[Co][3][3][C]: K = (kh, kw, ci), k_in = ci.  For
-// a matmul A[M][K] B[K][N] it rounds A and transposes B to Bt[N][K].
+//   stage 0     one launch that stages both operands: zero-pad k_in to a
+//               multiple of the K tile BK and cut it into blocks of BK (the
+//               block index outermost, the BK elements innermost: k-blocked),
+//               put the dims indexed only by reduced vars next, pad every dim
+//               that a view can index out of range with a zero halo, and
+//               round to bf16/fp16 (RNE) or keep float;
+//   stage 1     an implicit-GEMM tile of csrc/kernels/igemm_tile.h: every
+//               operand access is row_base(row) + outer_offset(k_out) + k_in
+//               with no bounds checks, loaded as 16-byte buffer loads.
+// For a 3x3 pad-1 convolution X[N][C][H][W] * W[Co][C][3][3] with BK 32 this
+// stages X'[C/32][N][H+2][W+2][32] and W'[C/32][3][3][Co][32]: K = (block,
+// kh, kw, ci).  Where the outer-K positions of a block only shift the input
+// rows (the window taps), the halo form of the tile loads those rows once
+// per block.  For a matmul A[M][K] B[K][N] it stages A'[K/BK][M][BK] and the
+// transposed Bt'[K/BK][N][BK].
 //
 // The staged buffers are backend scratch.  Out-of-range view reads still
 // read 0 (they land in the halo), as `cond ? x : 0` in the reference C
