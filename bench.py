@@ -391,17 +391,29 @@ def main():
     for _ in range(args.warmup):
         cc.run([b.ptr for b in io], sync=False)
     hip.synchronize()
+    # the timed steps, bracketed by HIP events on the plan's stream as well
+    stream = hip.compiled_stream(cc)
+    e0, e1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
+    e0.record(stream)
     for _ in range(args.steps):
         cc.run([b.ptr for b in io], sync=False)
+    e1.record(stream)
     hip.synchronize()
     el = time.perf_counter() - t0
     ms = el / args.steps * 1e3
+    gpu_step_ms = e0.elapsed_ms(e1) / args.steps
 
-    # dominant kernel: its own launches timed with HIP events on its stream
+    # dominant kernel: a one-kernel plan is timed by the events around the
+    # timed steps; otherwise its own launches are timed with HIP events on
+    # the plan's stream (median of three runs)
     key = "bytes" if bound == "hbm" else "flops"
     dom = max(range(len(st)), key=lambda i: st[i][key])
-    kern_ms = hip.time_stage_ms(cc, io, dom, warmup=3, iters=max(args.steps, 20))
+    if len(st) == 1:
+        kern_ms = gpu_step_ms
+    else:
+        kern_ms = sorted(hip.time_stage_ms(cc, io, dom, warmup=3, iters=max(args.steps, 20))
+                         for _ in range(3))[1]
     plan_ms = hip.time_ms(cc, io, warmup=3, iters=max(args.steps, 20))
     if bound == "hbm":
         value = bytes_step * args.steps / el / 1e9
@@ -436,6 +448,7 @@ def main():
         "config": dict(cfg, kernels=[s["kernel"] + ":" + s["kind"] for s in st],
                        algorithmic_bytes=bytes_step, algorithmic_flops=flops_step,
                        dominant_kernel=st[dom]["kernel"], kernel_ms=round(kern_ms, 5),
+                       gpu_ms_per_step=round(gpu_step_ms, 5),
                        plan_ms=round(plan_ms, 5), max_err_vs_f64=err),
         "roofline": roof,
     }
