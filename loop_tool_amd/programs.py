@@ -54,6 +54,15 @@ def matmul(L, M, N, K):
     return C, [(M, K), (K, N)]
 
 
+def bmm(L, Bn, M, N, K):
+    """C[b,m,n] = sum_k A[b,m,k] * B[b,k,n]: a batch of matmuls sharing b."""
+    b, m, n, k = L.Symbol("B"), L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
+    A = L.Tensor(Bn, M, K)
+    B = L.Tensor(Bn, K, N)
+    C = (A.as_(b, m, k) * B.as_(b, k, n)).sum(k)
+    return C, [(Bn, M, K), (Bn, K, N)]
+
+
 def softmax(L, R, Cn):
     """Row softmax over C: max-reduce, subtract, exp, sum-reduce, divide."""
     r, c = L.Symbol("R"), L.Symbol("C")
@@ -118,7 +127,7 @@ def add_broadcast(L, M, N):
 def reference_output(name, inputs, **kw):
     """float64 evaluation of a program (closed forms, not the oracle)."""
     x = [np.asarray(a, np.float64) for a in inputs]
-    if name == "matmul":
+    if name in ("matmul", "bmm"):
         return x[0] @ x[1]
     if name == "softmax":
         e = np.exp(x[0] - x[0].max(axis=1, keepdims=True))

@@ -90,6 +90,7 @@ CONTRACT = [
     ("conv2d", dict(N=4, Ci=64, Co=128, H=28, W=28)),
     ("conv2d", dict(N=1, Ci=32, Co=8, H=9, W=12, pad=2, k=5)),
     ("conv2d", dict(N=1, Ci=40, Co=24, H=5, W=300)),
+    ("bmm", dict(Bn=3, M=100, N=72, K=96)),
 ]
 
 
@@ -98,7 +99,7 @@ def ref_kw(name, kw):
 
 
 def contract_inputs(name, shapes):
-    if name == "matmul":
+    if name in ("matmul", "bmm"):
         return [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
     fan = shapes[1][1] * shapes[1][2] * shapes[1][3]
     return [P.normal(4, shapes[0], fan ** -0.5), P.normal(5, shapes[1], fan ** -0.5)]

@@ -38,6 +38,7 @@ PROGRAMS = [
     ("concat2d", dict(B=3, M0=5, M1=4), lambda s: [P.normal(8, s[0]), P.normal(9, s[1])]),
     ("transpose2d", dict(M=7, N=5), lambda s: [P.normal(10, s[0])]),
     ("add_broadcast", dict(M=6, N=9), lambda s: [P.normal(11, s[0]), P.normal(12, s[1])]),
+    ("bmm", dict(Bn=2, M=9, N=7, K=11), lambda s: [P.uniform(13, s[0]), P.uniform(14, s[1])]),
 ]
 
 
