@@ -46,7 +46,7 @@ def normal(seed, shape, std=1.0):
 
 # ---------------------------------------------------------------- programs --
 def matmul(L, M, N, K):
-    """C[m,n] = sum_k A[m,k] * B[k,n]  (config 1 of the benchmark suite)."""
+    """C[m,n] = sum_k A[m,k] * B[k,n]."""
     m, n, k = L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
     A = L.Tensor(M, K)
     B = L.Tensor(K, N)

@@ -58,7 +58,7 @@ def build_oracle(force=False):
 
 def build_reference(force=False, jobs=8):
     if not os.path.isdir(os.path.join(REF, "src")):
-        return None  # not available (e.g. on the GPU box); prebuilt copy is used
+        return None  # no reference sources here: a prebuilt copy is used if present
     os.makedirs(os.path.join(REF_OUT, "obj"), exist_ok=True)
     flags = ["-std=c++17", "-O2", "-fPIC", "-fvisibility=hidden",
              "-fvisibility-inlines-hidden", "-I" + os.path.join(REF, "include"),
