@@ -1,7 +1,8 @@
 """Benchmark of the loop_tool hip backend on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--nest softmax|matmul|conv2d|conv2d_bf16|maxpool|copy|batched|serving]
+                    [--nest softmax|matmul|matmul_f32_<n>|matmul_bf16_<n>|conv2d|conv2d_bf16|
+                            maxpool|copy|batched|serving]
 
 Default workload: FP32 row softmax over an 8192 x 8192 tensor (max-reduce,
 subtract, exp, sum-reduce, divide over the columns), written with the lazy
@@ -59,6 +60,11 @@ def workload(lt, name):
         out, shapes = P.matmul(lt, 128, 128, 128)
         gen = lambda: [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
         return out, shapes, gen, dict(workload="fp32 matmul M=N=K=128"), "mfma", "f32", "hip"
+    if name.startswith("matmul_f32"):
+        n = int(name.split("_")[-1]) if name.count("_") == 2 else 4096
+        out, shapes = P.matmul(lt, n, n, n)
+        gen = lambda: [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
+        return out, shapes, gen, dict(workload="matmul fp32, M=N=K=%d" % n), "mfma", "f32", "hip"
     if name.startswith("matmul_bf16"):
         n = int(name.split("_")[-1]) if name.count("_") == 2 else 8192
         out, shapes = P.matmul(lt, n, n, n)
@@ -109,7 +115,7 @@ def cpu_baseline(name, inputs_np, shapes, seconds, bytes_per_step, flops_per_ste
         ins = [inputs_np[0][:rows]]
         scale = rows / shapes[0][0]
         sample = "%d of %d rows" % (rows, shapes[0][0])
-    elif name.startswith("matmul_bf16"):
+    elif name.startswith("matmul_bf16") or name.startswith("matmul_f32"):
         rows = 16
         M, K = shapes[0]
         out, sh = P.matmul(L, rows, shapes[1][1], K)
@@ -380,7 +386,7 @@ def main():
         err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
     elif args.nest == "copy":
         err = float(np.abs(y - inputs_np[0].ravel()).max())
-    elif args.nest.startswith("matmul_bf16"):
+    elif args.nest.startswith("matmul_bf16") or args.nest.startswith("matmul_f32"):
         ref = P.reference_output("matmul", [inputs_np[0][:64], inputs_np[1]]).ravel()
         err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
     else:
