@@ -241,3 +241,22 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
         return u.astype(np.uint32).view(np.float32)
     f64_bf = P.reference_output(name, [bf16(a) for a in inputs], **ref_kw(name, kw)).ravel()
     assert normwise(ys[0], f64_bf) <= 1e-5
+
+
+def test_pointwise_past_2_to_the_30_elements(lt, hip_dev):
+    """Buffers of more than 2^30 elements take the 64-bit index form of the
+    pointwise lowering (32-bit offsets are used only below that size): a
+    broadcast add over 32769 x 32768 floats must equal numpy's float32 add bit
+    for bit everywhere, including past element 2^31 / 4 of the output."""
+    M, N = 32769, 32768
+    assert M * N > 1 << 30
+    out, shapes = P.add_broadcast(lt, M, N)
+    a = ((np.arange(M * N, dtype=np.int64) % 4093) - 2046).astype(np.float32) * 0.25
+    b = (np.arange(N, dtype=np.int64) % 17).astype(np.float32) * 0.5
+    t = out.loop_tree
+    cc = lt.compile(t, "hip")
+    assert cc.string_property("kinds") == "pointwise"
+    y = np.zeros(M * N, np.float32)
+    cc.run([a, b, y])
+    expect = (a.reshape(M, N) + b).ravel()
+    assert np.array_equal(y, expect)
