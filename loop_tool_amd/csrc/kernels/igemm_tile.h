@@ -190,9 +190,10 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 // 2^31 bytes (pbytes, qbytes: operand sizes in bytes) with KI % BK == 0, or,
 // with KTAIL, KI % 8 == 0: the last K tile then zeroes the 16-byte chunks
 // past KI (they hold the next row's values, or the buffer range check's 0).
-//  * WGM x WGN waves (4 or 8) split the tile; each wave owns
+//  * WGM x WGN waves (2, 4 or 8) split the tile; each wave owns
 //    (BM/WGM) x (BN/WGN) as 32x32 MFMA blocks.  Eight waves put two waves on
-//    every SIMD, so one wave's MFMAs cover the other's LDS and load waits.
+//    every SIMD, so one wave's MFMAs cover the other's LDS and load waits;
+//    two waves of larger blocks read fewer LDS bytes per MFMA.
 //  * Global loads are buffer loads: a 32-bit byte offset per row in a VGPR
 //    and the K tile's offset in an SGPR; rows outside the problem get an
 //    offset beyond the buffer, which the hardware reads as 0.
@@ -210,11 +211,8 @@ __device__ __forceinline__ void lt_igemm_tile(const G& g, const T* __restrict__ 
 //    fragment read (16 rows, one logical chunk) and of a tile store (whole
 //    rows) then touch 16 distinct 16-byte bank groups.
 // Each lane's working set: TM*TN*16 accumulators, 2*(TM+TN) fragments of 4
-// VGPRs and 2*(VP+VQ) load registers of 4.
-template <int BM, int BN, int BK, bool SWZ>
-__host__ __device__ constexpr int lt_igemm_pipe_lds_bytes() {
-  return 2 * (BM + BN) * (SWZ ? BK : BK + 8) * 2;
-}
+// VGPRs and 2*(VP+VQ) load registers of 4.  LDS: 2 * (BM + BN) rows of BK
+// (SWZ) or BK + 8 16-bit elements.
 
 template <int BK, bool SWZ>
 __device__ __forceinline__ int lt_ig_lds_off(int row, int chunk) {
@@ -559,11 +557,7 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
 // per K tile.  Per k16 step each lane reads 8 consecutive k of its fragment
 // row (two 16-byte LDS reads): lanes 0-31 hold k 0..7 and lanes 32-63 k 8..15
 // of the step, so MFMA m (m = 0..7) adds the products of k = m and k = 8 + m.
-template <int BM, int BN, int BK>
-__host__ __device__ constexpr int lt_igemm_f32_lds_bytes() {
-  return 2 * (BM + BN) * (BK + 4) * 4;
-}
-
+// LDS: 2 * (BM + BN) rows of BK + 4 floats.
 template <int BM, int BN, int BK, bool FULL, int WGM, int WGN, class G>
 __device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __restrict__ P,
                                                   const float* __restrict__ Q,
