@@ -66,7 +66,7 @@ def knob_candidates(kind):
     if kind == "contract_f32":  # staged float tile, or the gathered tile
         return ["tile=%dx%d" % t for t in ((64, 64), (128, 64), (64, 128), (128, 128))] + [
             "contract=staged", "contract=staged tile=128x64", "contract=staged waves=8",
-            "contract=gather"]
+            "contract=staged streamk=0", "contract=staged streamk=1", "contract=gather"]
     if kind.startswith("contract"):  # staged 16-bit operands
         tiles = ["tile=128x256", "tile=256x128", "tile=128x128 bk=64", "tile=256x256 bk=64",
                  "tile=64x128", "tile=128x64"]

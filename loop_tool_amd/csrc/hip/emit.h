@@ -59,6 +59,9 @@ struct Knobs {
   int stage_blocked = -1;      // "stage=blocked|flat": k-blocked operand staging
                                // (-1 = auto: blocked when k_in spans several K tiles)
   int kb_outer = -1;           // "kb=outer|inner": order of the K blocks
+  int streamk = -1;            // "streamk=0|1": float contractions share K tiles
+                               // evenly over the workgroups (-1 = auto: when
+                               // the tiles fill the workgroup slots unevenly)
   int igemm_halo = -1;         // "igemm=halo" (1) / "igemm=halo1" (2, one halo
                                // buffer) / "igemm=pipe" (0): staged 16-bit
                                // contractions load the Q rows of all taps of a

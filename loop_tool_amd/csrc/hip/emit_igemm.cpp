@@ -586,6 +586,18 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
                         : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;
+  const int64_t gx = (c.ND + BN - 1) / BN, gy = (c.MD + BM - 1) / BM;
+  ASSERT(gx < (1LL << 31) && gy < 65536 && c.BD < 65536) << "contraction grid too large";
+  // stream-K (float tile): when the tiles fill the 2 x 256 workgroup slots
+  // unevenly, SK workgroups share all tiles' K tiles evenly instead; every
+  // range spans at least a whole tile's K, so a tile has at most two
+  // contributors, whose partial sums are added with float atomics to an
+  // output that the staging launch zeroes (two commutative adds: the result
+  // does not depend on their order)
+  const int64_t SK = 512, ntile = gx * gy, KTT = ko_n * (KIT / BK);
+  const int64_t sk_rounds = (ntile + SK - 1) / SK;
+  const bool streamk = f32 && pipe && c.BD == 1 && knobs.streamk != 0 && ntile >= SK &&
+                       (knobs.streamk == 1 || ntile * 10 < sk_rounds * SK * 9);
 
   // ---- buffers: one staged (16-bit) scratch buffer per operand ----
   int sbuf[2];
@@ -598,35 +610,19 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     plan.buffers.push_back({BufferSpec::SCRATCH, scratch_no++, words, nr});
   }
   const int out_buf = df.num_inputs;  // the single output
+  // stream-K: the tiles that two workgroups share (a range boundary falls
+  // inside their K); the staging launch zeroes their output blocks
+  std::vector<int64_t> split_tiles;
+  if (streamk)
+    for (int64_t v = 1; v < SK; ++v) {
+      const int64_t b = ntile * KTT * v / SK;
+      if (b % KTT && (split_tiles.empty() || split_tiles.back() != b / KTT))
+        split_tiles.push_back(b / KTT);
+    }
+  const int64_t nzero = (int64_t)split_tiles.size();  // one workgroup per tile
 
   std::stringstream k;
   k << kernel_prelude() << igemm_tile_source() << "\n";
-  // ---- both staging passes in one launch: workgroups [0, nb[0]) stage
-  // operand 0, the rest operand 1 ----
-  int64_t nb[2];
-  for (int o = 0; o < 2; ++o)
-    nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt);
-  ASSERT(nb[0] + nb[1] < (1LL << 31)) << "staging grid too large";
-  k << "static constexpr unsigned lt_stage_blocks[2] = {" << nb[0] << "u, " << nb[1] << "u};\n"
-    << "extern \"C\" __global__ void __launch_bounds__(256) lt_stage0(const float* __restrict__ "
-       "in0, "
-    << elt << "* __restrict__ S0, const float* __restrict__ in1, " << elt
-    << "* __restrict__ S1) {\n"
-    << "  if (blockIdx.x < lt_stage_blocks[0]) lt_stage_op0(in0, S0, blockIdx.x);\n"
-    << "  else lt_stage_op1(in1, S1, blockIdx.x - lt_stage_blocks[0]);\n}\n";
-  {
-    Stage s;
-    s.kernel = "lt_stage0";
-    s.kind = "stage";
-    s.buffers = {c.ops[0].input_index, sbuf[0], c.ops[1].input_index, sbuf[1]};
-    s.writes = {sbuf[0], sbuf[1]};
-    s.grid[0] = (unsigned)(nb[0] + nb[1]);
-    s.block[0] = 256;
-    for (int o = 0; o < 2; ++o)
-      s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
-                 st[o].total * EB;
-    plan.stages.push_back(s);
-  }
   // geometry of the tile
   k << "struct LtGeo {\n"
     << "  static constexpr int KO = " << ko_n << ", KI = " << KIT << ";\n"
@@ -652,13 +648,62 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << decls(c.J, "    ") << decompose("j", c.J, "    ") << "    return "
     << lin_over(c.ostride, c.J) << ";\n  }\n"
     << "};\n";
+  // ---- both staging passes in one launch: workgroups [0, nb[0]) stage
+  // operand 0, the rest operand 1 ----
+  int64_t nb[2];
+  for (int o = 0; o < 2; ++o)
+    nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt);
+  ASSERT(nb[0] + nb[1] + nzero < (1LL << 31)) << "staging grid too large";
+  k << "static constexpr unsigned lt_stage_blocks[2] = {" << nb[0] << "u, " << nb[1] << "u};\n"
+    << "extern \"C\" __global__ void __launch_bounds__(256) lt_stage0(const float* __restrict__ "
+       "in0, "
+    << elt << "* __restrict__ S0, const float* __restrict__ in1, " << elt
+    << "* __restrict__ S1" << (streamk ? ", float* __restrict__ zo" : "") << ") {\n"
+    << "  if (blockIdx.x < lt_stage_blocks[0]) lt_stage_op0(in0, S0, blockIdx.x);\n";
+  if (streamk) {
+    k << "  else if (blockIdx.x >= lt_stage_blocks[0] + lt_stage_blocks[1]) {\n"
+      << "    // zero a split tile's output block for the stream-K atomic adds\n"
+      << "    static constexpr unsigned split_[" << nzero << "] = {";
+    for (size_t q = 0; q < split_tiles.size(); ++q) k << (q ? ", " : "") << split_tiles[q] << "u";
+    k << "};\n"
+      << "    const unsigned t_ = split_[blockIdx.x - lt_stage_blocks[0] - lt_stage_blocks[1]];\n"
+      << "    const long i0 = (long)(t_ / " << gx << "u) * " << BM << ", j0 = (long)(t_ % " << gx
+      << "u) * " << BN << ";\n"
+      << "    LtGeo g;\n    g.zp = g.zq = g.zo = 0;\n"
+      << "    for (int e = threadIdx.x; e < " << BM * BN << "; e += 256) {\n"
+      << "      const long i = i0 + e / " << BN << ", j = j0 + e % " << BN << ";\n"
+      << "      if (i < LtGeo::MD && j < LtGeo::ND) zo[g.out_row(i) + g.out_col(j)] = 0.f;\n"
+      << "    }\n  }\n";
+  }
+  k << "  else lt_stage_op1(in1, S1, blockIdx.x - lt_stage_blocks[0]);\n}\n";
+  {
+    Stage s;
+    s.kernel = "lt_stage0";
+    s.kind = "stage";
+    s.buffers = {c.ops[0].input_index, sbuf[0], c.ops[1].input_index, sbuf[1]};
+    s.writes = {sbuf[0], sbuf[1]};
+    if (streamk) {
+      s.buffers.push_back(out_buf);
+      s.writes.push_back(out_buf);
+      s.bytes += nzero * BM * BN * 4;
+    }
+    s.grid[0] = (unsigned)(nb[0] + nb[1] + nzero);
+    s.block[0] = 256;
+    for (int o = 0; o < 2; ++o)
+      s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
+                 st[o].total * EB;
+    plan.stages.push_back(s);
+  }
   // eight waves: keep two workgroups per CU (four waves per SIMD, at most
   // 128 VGPRs each) when their LDS fits and a wave's accumulators take at
   // most 64 of those registers
   const bool two_wg = threads == 512 && 2 * lds <= 160 * 1024 &&
                       (BM / WGM) * (BN / WGN) / 64 <= 64;
   k << "extern \"C\" __global__ void __launch_bounds__(" << threads << ")"
-    << (two_wg ? " __attribute__((amdgpu_waves_per_eu(4)))" : "") << " lt_igemm0(const " << elt
+    << (two_wg    ? " __attribute__((amdgpu_waves_per_eu(4)))"
+        : streamk ? " __attribute__((amdgpu_waves_per_eu(2)))"  // two 4-wave workgroups per CU
+                  : "")
+    << " lt_igemm0(const " << elt
     << "* __restrict__ P, const " << elt << "* __restrict__ Q, float* __restrict__ out) {\n"
     << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << lds << "];\n"
     << "  LtGeo g;\n"
@@ -667,13 +712,28 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << "  g.zq = " << lin_over(SQ.scoef, c.B) << ";\n"
     << "  g.zo = " << lin_over(c.ostride, c.B) << ";\n"
     ;
-  auto gx = (c.ND + BN - 1) / BN, gy = (c.MD + BM - 1) / BM;
-  ASSERT(gx < (1LL << 31) && gy < 65536 && c.BD < 65536) << "contraction grid too large";
   const std::string i0 = "(long)blockIdx.y * " + std::to_string(BM);
   const std::string j0 =
       "(long)lt_igemm_swizzle(blockIdx.x, gridDim.x) * " + std::to_string(BN);
   const std::string ind = "  ";
-  if (f32)
+  if (streamk)
+    k << "  const unsigned long U_ = " << ntile * KTT << "ul;\n"
+      << "  const unsigned v_ = lt_igemm_swizzle(blockIdx.x, gridDim.x);\n"
+      << "  unsigned long u_ = U_ * v_ / " << SK << "ul;\n"
+      << "  const unsigned long ue_ = U_ * (v_ + 1) / " << SK << "ul;\n"
+      << "  while (u_ < ue_) {\n"
+      << "    const unsigned t_ = (unsigned)(u_ / " << KTT << "ul), k0_ = (unsigned)(u_ % " << KTT
+      << "ul);\n"
+      << "    const unsigned long r_ = " << KTT << "ul - k0_;\n"
+      << "    const unsigned n_ = (unsigned)(r_ < ue_ - u_ ? r_ : ue_ - u_);\n"
+      << "    lt_igemm_tile_f32<" << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, (long)(t_ / "
+      << gx << "u) * " << BM << ", (long)(t_ % " << gx << "u) * " << BN << ", smem, "
+      << SP.total * 4 << "u, " << SQ.total * 4 << "u, (int)k0_, (int)n_, n_ != " << KTT
+      << "u);\n"
+      << "    __syncthreads();  // the next tile rewrites this tile's LDS\n"
+      << "    u_ += n_;\n  }\n";
+  else if (f32)
     k << ind << "lt_igemm_tile_f32<" << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, " << i0
       << ", " << j0 << ", smem, " << SP.total * 4 << "u, " << SQ.total * 4 << "u);\n";
@@ -698,8 +758,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   g.kind = bf16 ? "contract_bf16" : f32 ? "contract_f32" : "contract_f16";
   g.buffers = {sbuf[c.po], sbuf[c.qo], out_buf};
   g.writes = {out_buf};
-  g.grid[0] = (unsigned)gx;
-  g.grid[1] = (unsigned)gy;
+  g.grid[0] = (unsigned)(streamk ? SK : gx);
+  g.grid[1] = (unsigned)(streamk ? 1 : gy);
   g.grid[2] = (unsigned)c.BD;
   g.block[0] = (unsigned)threads;
   g.flops = 2 * c.MD * c.ND * c.KD * c.BD;
@@ -715,6 +775,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
                       : std::string())
        << (blocked ? std::string(", k-blocked staging (kb ") + (kb_first ? "outer)" : "inner)")
                    : std::string())
+       << (streamk ? ", stream-K over " + std::to_string(SK) + " workgroups" : std::string())
        << (HL ? ", Q halo of " + std::to_string(HL) + " rows per K block" +
                     (HB == 1 ? " (one buffer)" : "")
               : std::string())

@@ -23,6 +23,9 @@
 //   bk=N           K tile (32 or 64) of 16-bit contractions
 //   contract=gather  contractions gather their operands in the tile loader
 //                  instead of staging them
+//   streamk=0|1    staged float contractions: stream-K work split over 512
+//                  workgroups off / on (default: on when the tiles fill the
+//                  workgroup slots unevenly)
 //   contract=staged  float contractions are staged (and run the pipelined
 //                  float tile) whatever their size; by default only nests of
 //                  2^28 or more multiply-adds are
@@ -183,6 +186,8 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       else if (tok == "mfma_f16") k.f16 = true;
       else if (tok == "contract=gather") k.gather = true;
       else if (tok == "contract=staged") k.staged = true;
+      else if (tok == "streamk=0") k.streamk = 0;
+      else if (tok == "streamk=1") k.streamk = 1;
       else if (tok == "igemm=plain") k.igemm_plain = true;
       else if (tok == "igemm=halo") k.igemm_halo = 1;
       else if (tok == "igemm=halo1") k.igemm_halo = 2;

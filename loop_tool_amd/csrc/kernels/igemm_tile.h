@@ -67,10 +67,12 @@ __host__ __device__ constexpr int lt_igemm_lds_bytes() {
 
 // Writes a wave's TM x TN accumulator blocks (rows from i0, columns from j0).
 // 32x32 accumulator layout: col = lane % 32, row = (r/4)*8 + (lane/32)*4 + r%4.
+// With `add`, the blocks are added to `out` with float atomics instead (a
+// tile whose K range two workgroups share; `out` starts at zero).
 template <int TM, int TN, bool FULL, class G>
 __device__ __forceinline__ void lt_igemm_store(const G& g, const lt_ig_f32x16 (&acc)[TM][TN],
                                                float* __restrict__ out, long i0, long j0,
-                                               int lane) {
+                                               int lane, bool add = false) {
 #pragma unroll
   for (int b = 0; b < TN; ++b) {
     const long j = j0 + b * 32 + (lane & 31);
@@ -81,8 +83,12 @@ __device__ __forceinline__ void lt_igemm_store(const G& g, const lt_ig_f32x16 (&
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const long i = i0 + a * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-        if (FULL || i < g.MD)
-          __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
+        if (FULL || i < g.MD) {
+          if (add)
+            unsafeAtomicAdd(out + g.out_row(i) + oj, acc[a][b][r]);
+          else
+            __builtin_nontemporal_store(acc[a][b][r], out + g.out_row(i) + oj);
+        }
       }
     }
   }
@@ -557,13 +563,15 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
 // per K tile.  Per k16 step each lane reads 8 consecutive k of its fragment
 // row (two 16-byte LDS reads): lanes 0-31 hold k 0..7 and lanes 32-63 k 8..15
 // of the step, so MFMA m (m = 0..7) adds the products of k = m and k = 8 + m.
-// LDS: 2 * (BM + BN) rows of BK + 4 floats.
+// LDS: 2 * (BM + BN) rows of BK + 4 floats.  kt0, nkt: the K tiles to run
+// (all by default); add: accumulate into `out` (see lt_igemm_store).
 template <int BM, int BN, int BK, bool FULL, int WGM, int WGN, class G>
 __device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __restrict__ P,
                                                   const float* __restrict__ Q,
                                                   float* __restrict__ out, long i0, long j0,
                                                   unsigned char* smem, unsigned pbytes,
-                                                  unsigned qbytes) {
+                                                  unsigned qbytes, int kt0 = 0, int nkt = -1,
+                                                  bool add = false) {
   constexpr int NT = 64 * WGM * WGN, LDR = BK + 4;
   constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
   constexpr int CPR = BK / 4, KK = BK / 16;   // 16-byte chunks per row, k16 steps
@@ -573,7 +581,7 @@ __device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __res
                 "tile too small for the workgroup");
   static_assert(KK % 2 == 0, "BK must be a multiple of 32");
   const int KIT = g.KI / BK;
-  const int KT = g.KO * KIT;
+  const int KT = nkt < 0 ? g.KO * KIT : nkt;  // K tiles kt0 .. kt0 + KT - 1
   float* sP = (float*)smem;      // [2][BM][LDR]
   float* sQ = sP + 2 * BM * LDR; // [2][BN][LDR]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -596,7 +604,7 @@ __device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __res
   }
   lt_ig_u32x4 r0p[VP], r0q[VQ], r1p[VP], r1q[VQ];
   auto load = [&](int kt, lt_ig_u32x4* rp, lt_ig_u32x4* rq) {
-    const int ko = kt / KIT, kb = (kt % KIT) * BK;
+    const int ko = (kt0 + kt) / KIT, kb = ((kt0 + kt) % KIT) * BK;
     const int po = (int)((g.p_koff(ko) + kb) * 4), qo = (int)((g.q_koff(ko) + kb) * 4);
 #pragma unroll
     for (int v = 0; v < VP; ++v)
@@ -678,7 +686,7 @@ __device__ __forceinline__ void lt_igemm_tile_f32(const G& g, const float* __res
   } else {
     step(kt, 0, r1p, r1q, r0p, r0q, false, false);
   }
-  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane, add);
 }
 
 #endif  // LT_IGEMM_TILE_H_

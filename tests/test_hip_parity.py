@@ -260,3 +260,53 @@ def test_pointwise_past_2_to_the_30_elements(lt, hip_dev):
     cc.run([a, b, y])
     expect = (a.reshape(M, N) + b).ravel()
     assert np.array_equal(y, expect)
+
+
+def run_plan(lt, out, inputs, knobs, backend="hip"):
+    t = out.loop_tree
+    if knobs:
+        t = t.annotate(t.roots[0], knobs)
+    cc = lt.compile(t, backend)
+    y = np.zeros(t.io_numel()[len(inputs)], np.float32)
+    cc.run(inputs + [y])
+    return y, cc
+
+
+def test_streamk_float_matmul_vs_f64(lt, hip_dev):
+    """576 tiles on 512 workgroup slots: the float tile runs stream-K (every
+    workgroup an equal share of all tiles' K tiles, tiles split between two
+    workgroups accumulated with float atomics into the zeroed output).
+    <= 1e-5 normwise vs float64, and two runs give the same bits."""
+    out, shapes = P.matmul(lt, 3072, 3072, 256)
+    inputs = [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
+    y, cc = run_plan(lt, out, inputs, "")
+    assert "stream-K" in cc.dump().splitlines()[0]
+    f64 = P.reference_output("matmul", inputs).ravel()
+    assert normwise(y, f64) <= 1e-5
+    y2, _ = run_plan(lt, out, inputs, "")
+    assert np.array_equal(y.view(np.uint32), y2.view(np.uint32))
+    y0, cc0 = run_plan(lt, out, inputs, "streamk=0")
+    assert "stream-K" not in cc0.dump().splitlines()[0]
+    assert normwise(y, y0) <= 1e-5
+
+
+def test_streamk_float_conv(lt, hip_dev):
+    """The float conv of the benchmark shape (784 tiles of 128 x 128): the
+    stream-K plan against the one-tile-per-workgroup plan (<= 1e-5
+    normwise, the float32 bar; they differ only in how split tiles' two
+    partial sums are rounded), repeatable to the bit, and against float64 on
+    a sample of output channels."""
+    out, shapes = P.conv2d(lt, 32, 128, 128, 56, 56)
+    std = 1.0 / np.sqrt(128 * 9)
+    inputs = [P.normal(4, shapes[0], std), P.normal(5, shapes[1], std)]
+    y, cc = run_plan(lt, out, inputs, "")
+    assert "stream-K" in cc.dump().splitlines()[0]
+    y2, _ = run_plan(lt, out, inputs, "")
+    assert np.array_equal(y.view(np.uint32), y2.view(np.uint32))
+    y0, _ = run_plan(lt, out, inputs, "streamk=0")
+    assert normwise(y, y0) <= 1e-5
+    # float64 for 8 output channels spread over the range
+    co = np.arange(0, 128, 16)
+    f64 = P.reference_output("conv2d", [inputs[0], inputs[1][co]])
+    got = y.reshape(32, 128, 56, 56)[:, co]
+    assert normwise(got.ravel(), f64.ravel()) <= 1e-5
