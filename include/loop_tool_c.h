@@ -112,7 +112,7 @@ int ltc_looptree_dump(const ltc_looptree *lt, char *buf, int cap);
 int ltc_looptree_size(const ltc_looptree *lt);          /* number of tree refs */
 int ltc_looptree_roots(const ltc_looptree *lt, int *out, int cap);
 int ltc_looptree_children(const ltc_looptree *lt, int ref, int *out, int cap);
-int ltc_looptree_parent(const ltc_looptree *lt, int ref);
+int ltc_looptree_parent(const ltc_looptree *lt, int ref); /* -1: a root; -2: error */
 int ltc_looptree_is_loop(const ltc_looptree *lt, int ref);
 int ltc_looptree_loop(const ltc_looptree *lt, int ref, int *var, int *var_depth,
                       int64_t *size, int64_t *tail);

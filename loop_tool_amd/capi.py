@@ -502,8 +502,11 @@ class LoopTree(_Handle):
         return self._ints(self.L.lib.ltc_looptree_children, ref)
 
     def parent(self, ref):
-        return self.L.check_int(self.L.lib.ltc_looptree_parent(self.h, ref)) \
-            if ref >= 0 else -1
+        """The parent's ref, -1 for a root (LoopTree::parent, ir.h)."""
+        r = self.L.lib.ltc_looptree_parent(self.h, ref)
+        if r < -1:
+            raise LoopToolError(self.L.error())
+        return r
 
     def is_loop(self, ref):
         return bool(self.L.check_int(self.L.lib.ltc_looptree_is_loop(self.h, ref)))

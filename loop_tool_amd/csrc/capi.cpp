@@ -356,7 +356,15 @@ int ltc_looptree_children(const ltc_looptree *lt, int ref, int *out, int cap) {
 }
 
 int ltc_looptree_parent(const ltc_looptree *lt, int ref) {
-  return guard_int([&] { return (int)lt->lt.parent(ref); });
+  // -1 is a root's parent (LoopTree::parent), so errors return -2
+  try {
+    return (int)lt->lt.parent(ref);
+  } catch (const std::exception &e) {
+    g_error = e.what();
+  } catch (...) {
+    g_error = "unknown error";
+  }
+  return -2;
 }
 
 int ltc_looptree_is_loop(const ltc_looptree *lt, int ref) {

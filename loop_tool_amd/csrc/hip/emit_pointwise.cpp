@@ -10,7 +10,10 @@
 // reference C backend.  Together with -ffp-contract=off this makes the path
 // bit-exact with the C oracle for + - * / min max and for the index-only
 // nests; exp/log/sqrt are evaluated in double and rounded to float as the C
-// backend does through libm.
+// backend does through libm.  One deliberate difference: a tail block inside
+// another loop's tail block starts at the iterations actually run there, not
+// at the loop's full size as in the C backend, whose index then runs past the
+// buffers (such schedules come from splitting one var three or more times).
 #include <algorithm>
 #include <functional>
 #include <memory>
@@ -346,8 +349,11 @@ struct PointwiseEmitter::Impl {
       b->open_child = blk;
       auto o = overrides;
       o[L.var] = (int)tail;
-      // the C backend pins the iterator of a tail block to the loop's size
-      iter_terms.emplace_back(L.var, Expr(L.size * L.inner));
+      // a tail block starts after the `size` full iterations run in this
+      // context.  The C backend pins it to the loop's own size instead
+      // (cpp.cpp:338-346), which differs inside another loop's tail block:
+      // there it indexes past the buffers and skips the last elements.
+      iter_terms.emplace_back(L.var, Expr(size * L.inner));
       chain(blk, n, i + 1, o, subst, acc);
       iter_terms.pop_back();
       b->open_child = saved;

@@ -256,6 +256,9 @@ class CCodegen : public Compiler {
     ss << prefix << "}\n";
     if (tail > 0) {
       ss << prefix << "{ // " << lt.ir.var(loop.var).name() << " tail\n";
+      // pinned to the loop's own size even where an override shrank it, as
+      // cpp.cpp:340-341 does: kept, since this file restates the reference
+      // (inside another tail block that index runs past the buffers)
       ss << indent(ref, 1) << "int64_t " << it << " = " << loop.size << "L;\n";
       for (const auto &b : tail_body) ss << b;
       ss << prefix << "}\n";
