@@ -660,7 +660,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << elt << "* __restrict__ S0, const float* __restrict__ in1, " << elt
     << "* __restrict__ S1" << (streamk ? ", float* __restrict__ zo" : "") << ") {\n"
     << "  if (blockIdx.x < lt_stage_blocks[0]) lt_stage_op0(in0, S0, blockIdx.x);\n";
-  if (streamk) {
+  if (nzero) {
     k << "  else if (blockIdx.x >= lt_stage_blocks[0] + lt_stage_blocks[1]) {\n"
       << "    // zero a split tile's output block for the stream-K atomic adds\n"
       << "    static constexpr unsigned split_[" << nzero << "] = {";
@@ -682,7 +682,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     s.kind = "stage";
     s.buffers = {c.ops[0].input_index, sbuf[0], c.ops[1].input_index, sbuf[1]};
     s.writes = {sbuf[0], sbuf[1]};
-    if (streamk) {
+    if (streamk) {  // (unused by the launch when no tile is shared)
       s.buffers.push_back(out_buf);
       s.writes.push_back(out_buf);
       s.bytes += nzero * BM * BN * 4;

@@ -76,3 +76,18 @@ def test_staged_layout_of_conv_operands(lt):
     d = lt.compile(t.annotate(t.roots[0], "stage=flat"), "hip_bf16").dump()
     assert "K=9x128" in d and "staged P 147456 and Q 13778944 elements" in d
     assert "k-blocked" not in d.splitlines()[0]
+
+
+def test_streamk_plans(lt):
+    """The float conv of the benchmark shape splits 784 tiles over 512
+    workgroups and zeroes only the shared tiles' output blocks at staging;
+    forced on a shape whose range boundaries all fall between tiles, the
+    plan has no zeroing branch (and still compiles)."""
+    out, _ = P.conv2d(lt, 32, 128, 128, 56, 56)
+    d = lt.compile(out.loop_tree, "hip").dump()
+    assert "stream-K over 512 workgroups" in d.splitlines()[0]
+    n_split = int(d.split("split_[", 1)[1].split("]", 1)[0])
+    assert 0 < n_split < 512
+    t = P.matmul(lt, 2048, 2048, 96)[0].loop_tree
+    d = lt.compile(t.annotate(t.roots[0], "contract=staged tile=64x64 streamk=1"), "hip").dump()
+    assert "stream-K" in d.splitlines()[0] and "split_[" not in d

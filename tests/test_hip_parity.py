@@ -310,3 +310,16 @@ def test_streamk_float_conv(lt, hip_dev):
     f64 = P.reference_output("conv2d", [inputs[0], inputs[1][co]])
     got = y.reshape(32, 128, 56, 56)[:, co]
     assert normwise(got.ravel(), f64.ravel()) <= 1e-5
+
+
+def test_streamk_whole_tiles_only_bit_identical(lt, hip_dev):
+    """1024 tiles of 64 x 64 over 512 workgroups: every range boundary falls
+    between tiles, so forced stream-K shares no tile (no zeroing, no atomic
+    adds) and equals the one-tile-per-workgroup plan to the bit."""
+    out, shapes = P.matmul(lt, 2048, 2048, 96)
+    inputs = [P.uniform(1, shapes[0]), P.uniform(2, shapes[1])]
+    y, cc = run_plan(lt, out, inputs, "contract=staged tile=64x64 streamk=1")
+    assert "stream-K" in cc.dump().splitlines()[0] and "split_[" not in cc.dump()
+    y0, _ = run_plan(lt, out, inputs, "contract=staged tile=64x64 streamk=0")
+    assert np.array_equal(y.view(np.uint32), y0.view(np.uint32))
+    assert normwise(y, P.reference_output("matmul", inputs).ravel()) <= 1e-5
