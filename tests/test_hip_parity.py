@@ -323,3 +323,43 @@ def test_streamk_whole_tiles_only_bit_identical(lt, hip_dev):
     y0, _ = run_plan(lt, out, inputs, "contract=staged tile=64x64 streamk=0")
     assert np.array_equal(y.view(np.uint32), y0.view(np.uint32))
     assert normwise(y, P.reference_output("matmul", inputs).ravel()) <= 1e-5
+
+
+def _bf16_round(a):
+    u = a.astype(np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+KNOB_OPTIONS = [["", "tile=64x64", "tile=128x128", "tile=64x128", "tile=128x256",
+                 "tile=256x256 bk=64"],
+                ["", "waves=2", "waves=8"], ["", "lds=swz"], ["", "stage=flat"],
+                ["", "kb=inner", "kb=outer"],
+                ["", "igemm=plain", "igemm=pipe", "igemm=halo", "igemm=halo1"],
+                ["", "contract=staged", "contract=gather"], ["", "streamk=1", "streamk=0"]]
+
+
+def random_knob_cases(n=48, seed=21):
+    import random
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        name, kw = rng.choice(CONTRACT)
+        knobs = " ".join(x for x in (rng.choice(o) for o in KNOB_OPTIONS) if x)
+        out.append((name, kw, rng.choice(["hip", "hip_bf16"]), knobs))
+    return out
+
+
+@pytest.mark.parametrize("name,kw,backend,knobs", random_knob_cases(),
+                         ids=lambda v: v.replace(" ", "_") if isinstance(v, str) else "")
+def test_random_knob_sets(lt, hip_dev, name, kw, backend, knobs):
+    """Seeded random combinations of every contraction knob (tile, waves,
+    LDS layout, staging layout and K-block order, tile form, staged or
+    gathered, stream-K): float operands within 1e-5 of float64, 16-bit
+    operands within 1e-5 of float64 of the rounded operands."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    y, cc = run_backend(lt, out, shapes, inputs, backend, annotate=knobs)
+    ops = inputs if backend == "hip" else [_bf16_round(a) for a in inputs]
+    f64 = P.reference_output(name, ops, **ref_kw(name, kw)).ravel()
+    assert normwise(y, f64) <= 1e-5, (cc.string_property("kinds"), cc.dump().splitlines()[0])
