@@ -177,6 +177,13 @@ std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
 Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
   Knobs k;
   k.bf16 = bf16_variant;
+  // "key=value" tokens whose key is a knob of this backend must carry a
+  // valid value; other tokens are annotations for other consumers
+  auto number = [](const std::string &tok, const std::string &v) {
+    ASSERT(!v.empty() && v.size() < 6 && v.find_first_not_of("0123456789") == std::string::npos)
+        << "knob " << tok << ": expected a number";
+    return std::stoi(v);
+  };
   auto scan = [&](const std::string &annot) {
     std::stringstream ss(annot);
     std::string tok;
@@ -184,30 +191,50 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       if (tok == "hip:plain") k.force_plain = true;
       else if (tok == "mfma_bf16") k.bf16 = true;
       else if (tok == "mfma_f16") k.f16 = true;
-      else if (tok == "contract=gather") k.gather = true;
-      else if (tok == "contract=staged") k.staged = true;
-      else if (tok == "streamk=0") k.streamk = 0;
-      else if (tok == "streamk=1") k.streamk = 1;
-      else if (tok == "igemm=plain") k.igemm_plain = true;
-      else if (tok == "igemm=halo") k.igemm_halo = 1;
-      else if (tok == "igemm=halo1") k.igemm_halo = 2;
-      else if (tok == "igemm=pipe") k.igemm_halo = 0;
-      else if (tok == "lds=swz") k.lds_swz = 1;
-      else if (tok == "lds=pad") k.lds_swz = 0;
-      else if (tok == "stage=blocked") k.stage_blocked = 1;
-      else if (tok == "stage=flat") k.stage_blocked = 0;
-      else if (tok == "kb=outer") k.kb_outer = 1;
-      else if (tok == "kb=inner") k.kb_outer = 0;
-      else if (tok.rfind("waves=", 0) == 0) k.waves = std::stoi(tok.substr(6));
-      else if (tok.rfind("bk=", 0) == 0) k.bk = std::stoi(tok.substr(3));
-      else if (tok.rfind("block=", 0) == 0) k.block = std::stoi(tok.substr(6));
-      else if (tok.rfind("vec=", 0) == 0) k.vec = std::stoi(tok.substr(4));
-      else if (tok.rfind("tile=", 0) == 0) {
-        auto x = tok.find('x', 5);
-        if (x != std::string::npos) {
-          k.tile_m = std::stoi(tok.substr(5, x - 5));
-          k.tile_n = std::stoi(tok.substr(x + 1));
-        }
+      auto eq = tok.find('=');
+      if (eq == std::string::npos) continue;
+      const std::string key = tok.substr(0, eq), v = tok.substr(eq + 1);
+      auto bad = [&]() { ASSERT(0) << "knob " << tok << ": unknown value"; };
+      if (key == "contract") {
+        if (v == "gather") k.gather = true;
+        else if (v == "staged") k.staged = true;
+        else bad();
+      } else if (key == "streamk") {
+        if (v == "0" || v == "1") k.streamk = v == "1";
+        else bad();
+      } else if (key == "igemm") {
+        if (v == "plain") k.igemm_plain = true;
+        else if (v == "halo") k.igemm_halo = 1;
+        else if (v == "halo1") k.igemm_halo = 2;
+        else if (v == "pipe") k.igemm_halo = 0;
+        else bad();
+      } else if (key == "lds") {
+        if (v == "swz" || v == "pad") k.lds_swz = v == "swz";
+        else bad();
+      } else if (key == "stage") {
+        if (v == "blocked" || v == "flat") k.stage_blocked = v == "blocked";
+        else bad();
+      } else if (key == "kb") {
+        if (v == "outer" || v == "inner") k.kb_outer = v == "outer";
+        else bad();
+      } else if (key == "waves") {
+        k.waves = number(tok, v);
+      } else if (key == "bk") {
+        k.bk = number(tok, v);
+        ASSERT(k.bk == 32 || k.bk == 64) << "knob " << tok << ": bk=N must be 32 or 64";
+      } else if (key == "block") {
+        k.block = number(tok, v);
+      } else if (key == "vec") {
+        k.vec = number(tok, v);
+        ASSERT(k.vec == 1 || k.vec == 2 || k.vec == 4) << "knob " << tok << ": vec=N must be 1, 2 or 4";
+      } else if (key == "tile") {
+        auto x = v.find('x');
+        ASSERT(x != std::string::npos) << "knob " << tok << ": expected tile=MxN";
+        k.tile_m = number(tok, v.substr(0, x));
+        k.tile_n = number(tok, v.substr(x + 1));
+        for (int d : {k.tile_m, k.tile_n})
+          ASSERT(d >= 32 && d <= 256 && d % 32 == 0)
+              << "knob " << tok << ": tile sides must be multiples of 32 in [32, 256]";
       }
     }
   };

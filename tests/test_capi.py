@@ -47,7 +47,6 @@ def test_errors_cross_the_boundary_as_codes(lt):
     import loop_tool_amd as lt_mod
     with pytest.raises(lt_mod.LoopToolError, match="couldn't find backend"):
         lt.set_default_backend("nonexistent")
-    assert lt.compile.__doc__ is None or True
     s = lt.Symbol("X")
     with pytest.raises(lt_mod.LoopToolError):
         lt.Tensor(s).numpy()  # unbound size

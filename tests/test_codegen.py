@@ -91,3 +91,23 @@ def test_streamk_plans(lt):
     t = P.matmul(lt, 2048, 2048, 96)[0].loop_tree
     d = lt.compile(t.annotate(t.roots[0], "contract=staged tile=64x64 streamk=1"), "hip").dump()
     assert "stream-K" in d.splitlines()[0] and "split_[" not in d
+
+
+@pytest.mark.parametrize("knob,msg", [("waves=3", "waves=N must be 2, 4 or 8"),
+                                      ("tile=100x64", "multiples of 32"),
+                                      ("tile=abc", "expected tile=MxN"),
+                                      ("bk=7", "bk=N must be 32 or 64"),
+                                      ("block=3", "block=N must be a multiple of 64"),
+                                      ("vec=3", "vec=N must be 1, 2 or 4"),
+                                      ("igemm=foo", "unknown value"),
+                                      ("streamk=2", "unknown value"),
+                                      ("waves=-1", "expected a number")])
+def test_bad_knob_values_are_errors(lt, knob, msg):
+    """A token whose key is a knob of the hip backend must carry a valid
+    value; annotations meant for other consumers pass through."""
+    import loop_tool_amd as lt_mod
+    out, _ = P.matmul(lt, 64, 64, 64)
+    t = out.loop_tree
+    with pytest.raises(lt_mod.LoopToolError, match=msg):
+        lt.compile(t.annotate(t.roots[0], knob), "hip")
+    lt.compile(t.annotate(t.roots[0], "parallel unroll=4 tile=64x64"), "hip")
