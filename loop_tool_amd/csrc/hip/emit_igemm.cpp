@@ -510,7 +510,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     if (knobs.bk > 0) BK = knobs.bk;
   }
   if (BK != 32 && BK != 64) return false;
-  if (BM % 64 || BN % 64 || BM > 256 || BN > 256 || BM * BN > 256 * 256) return false;
+  if (BM % 64 || BN % 64 || BM > 512 || BN > 512 || BM * BN > 256 * 256) return false;
   if (2 * (BM + BN) * (BK + 16 / EB) * EB > 160 * 1024) return false;  // LDS of a CU
   const int64_t KIP = (KI + BK - 1) / BK * BK;
   // k-blocked staging: K tiles of BK over k_in become separate outer-K

@@ -233,8 +233,8 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
         k.tile_m = number(tok, v.substr(0, x));
         k.tile_n = number(tok, v.substr(x + 1));
         for (int d : {k.tile_m, k.tile_n})
-          ASSERT(d >= 32 && d <= 256 && d % 32 == 0)
-              << "knob " << tok << ": tile sides must be multiples of 32 in [32, 256]";
+          ASSERT(d >= 32 && d <= 512 && d % 32 == 0)
+              << "knob " << tok << ": tile sides must be multiples of 32 in [32, 512]";
       }
     }
   };

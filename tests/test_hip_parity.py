@@ -210,7 +210,8 @@ HALO = [
 
 
 @pytest.mark.parametrize("name,kw", HALO, ids=["%s%d" % (c[0], i) for i, c in enumerate(HALO)])
-@pytest.mark.parametrize("tile", ["", "tile=128x128", "tile=64x128", "tile=128x256 waves=4"])
+@pytest.mark.parametrize("tile", ["", "tile=128x128", "tile=64x128", "tile=128x256 waves=4",
+                                  "tile=128x512"])
 def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     """The halo form of the staged tile (the Q rows of all window taps of a
     channel block loaded once into LDS, in two buffers or one) computes the
@@ -220,6 +221,8 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     accumulation error of float64 over the bf16-rounded operands.  Covers
     partial tiles, stride 2, a 5x5 window, rows longer than a tile, pad 0
     and channel counts that are not a multiple of the K block."""
+    if tile == "tile=128x512" and kw.get("W") == 300:
+        pytest.skip("two halo buffers of 1122 rows exceed the LDS of a CU")
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     ys = []
