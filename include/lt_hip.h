@@ -75,6 +75,14 @@ int lt_hip_launch(void *function, unsigned grid_x, unsigned grid_y, unsigned gri
                   unsigned block_x, unsigned block_y, unsigned block_z,
                   unsigned shared_bytes, void *stream, void **args);
 
+/* Workgroups of `function` (with `block` threads and `shared_bytes` of
+ * dynamic LDS) that the current device holds at once: per-CU occupancy times
+ * the CU count.  No counterpart in the reference, whose CUDA backend sized
+ * its 1-D grids by the nest alone (cuda.cpp:776-792); persistent launches
+ * here size their grids with it. */
+int lt_hip_max_resident_blocks(void *function, unsigned block, unsigned shared_bytes,
+                               int *blocks);
+
 /* ---- memory ----------------------------------------------------------------
  * Replaces cudaMallocManaged / cudaFree (cuda.cpp:800-816).  Device memory is
  * plain hipMalloc (HBM); host staging is done by the caller with the copies

@@ -38,8 +38,8 @@ for _n, _a in (("lth_batch_set_operands", [_P, C.POINTER(_P), C.POINTER(_P), C.P
 _L.lth_batch_free.restype = None
 _L.lth_batch_free.argtypes = [_P]
 
-TILES = [(64, 64), (128, 64), (64, 128), (128, 128), (256, 128), (128, 256)]
-TILE_K = [32, 32, 32, 64, 32, 32]
+TILES = [(64, 64), (128, 64), (64, 128), (128, 128), (256, 128), (128, 256), (128, 128)]
+TILE_K = [32, 32, 32, 64, 32, 32, 32]
 DEFAULT_CFG = 3
 
 
@@ -103,10 +103,10 @@ class Batch:
     def run_plain(self, cfg=DEFAULT_CFG):
         _chk(_L.lth_batch_run_plain(self.h, cfg, self.stream))
 
-    def run_grouped(self, grid=512):
+    def run_grouped(self, grid=0):
         _chk(_L.lth_batch_run_grouped(self.h, grid, self.stream))
 
-    def capture(self, grid=512):
+    def capture(self, grid=0):
         _chk(_L.lth_batch_capture(self.h, grid, self.stream))
 
     def replay(self):
@@ -168,7 +168,7 @@ def time_ms(fn, iters=10, stream=None):
     return e0.elapsed_ms(e1) / iters
 
 
-def tune_buckets(batch, grid=512, iters=3, store=None, L=None, rounds=1):
+def tune_buckets(batch, grid=0, iters=3, store=None, L=None, rounds=1):
     """Choose a tile configuration per shape bucket *in the context of the
     whole batch*: start from the best uniform configuration, then, bucket by
     bucket (most flops first), keep any configuration that makes the whole
@@ -205,7 +205,8 @@ def tune_buckets(batch, grid=512, iters=3, store=None, L=None, rounds=1):
                 if t < best * 0.99:
                     best, cfg = t, trial
     batch.set_configs(cfg)
-    report = dict(uniform_ms={"%dx%d" % TILES[c]: round(t, 4) for c, t in uniform.items()},
+    report = dict(uniform_ms={"%dx%dx%d" % (TILES[c] + (TILE_K[c],)): round(t, 4)
+                              for c, t in uniform.items()},
                   tuned_ms=round(best, 4), buckets={})
     for key in order:
         c = cfg[groups[key][0]]

@@ -8,9 +8,10 @@
 //            per-nest behaviour, one Compiled::run per matmul);
 //   grouped  one persistent launch over a work list holding every output
 //            tile of every GEMM, each with the tile configuration chosen for
-//            its shape bucket, ordered by decreasing cost so that the last
-//            tiles to start are the cheap ones; optionally captured once into
-//            a hipGraph and replayed.
+//            its shape bucket, ordered by decreasing cost and taken by the
+//            workgroups as a queue, so that the last tiles to start are the
+//            cheap ones; optionally captured once into a hipGraph and
+//            replayed.
 // The kernels are hand-written HIP (csrc/kernels/gemm.hip, on the tile of
 // csrc/kernels/igemm_tile.h) built by hipcc into kernels/gemm.hsaco next to
 // this library.
@@ -42,10 +43,14 @@ struct Item {
 };
 
 // tile configurations of csrc/kernels/gemm.hip (BM x BN x BK)
-constexpr int kNumCfg = 6;
-const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128};
-const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256};
-const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32};
+constexpr int kNumCfg = 7;
+const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128, 128};
+const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256, 128};
+const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32, 32};
+
+// LDS of the pipelined tile of configuration c (igemm_tile.h,
+// lt_igemm_lds_bytes: two stages of both operand tiles, rows padded by 16 B)
+int lds_bytes(int c) { return 2 * (kTileM[c] + kTileN[c]) * (kTileK[c] + 8) * 2; }
 
 void chk(int rc, const char *what) {
   ASSERT(rc == 0) << what << ": " << lt_hip_last_error();
@@ -92,6 +97,7 @@ struct lth_batch {
   std::vector<Desc> descs;
   std::vector<int> cfg;
   void *d_descs = nullptr, *d_items = nullptr;
+  void *d_queue = nullptr;  // grouped kernel's item queue: next item, workgroups done
   int n_items = 0;
   bool dirty = true;
   void *graph = nullptr;
@@ -125,6 +131,8 @@ struct lth_batch {
     chk(lt_hip_memcpy(d_descs, descs.data(), descs.size() * sizeof(Desc), 0), "descs copy");
     if (!sorted.empty())
       chk(lt_hip_memcpy(d_items, sorted.data(), sorted.size() * sizeof(Item), 0), "items copy");
+    if (!d_queue) chk(lt_hip_malloc(&d_queue, 2 * sizeof(int)), "queue");
+    chk(lt_hip_memset(d_queue, 0, 2 * sizeof(int)), "queue reset");
     n_items = (int)sorted.size();
     dirty = false;
     if (graph) {
@@ -136,12 +144,23 @@ struct lth_batch {
   void grouped(int grid, void *stream) {
     upload();
     if (!n_items) return;
-    void *fn = gemm_module().fn(std::string("lt_ggemm_") + tname());
+    // the smallest register class of grouped kernel covering the batch's
+    // configurations (gemm.hip: small 0-2, mid adds 6, the full kernel all)
+    int cls = 0;
+    for (int c : cfg) cls = std::max(cls, c <= 2 ? 0 : c == 6 ? 1 : 2);
+    const char *kname[3] = {"lt_ggemm_small_", "lt_ggemm_mid_", "lt_ggemm_"};
+    void *fn = gemm_module().fn(std::string(kname[cls]) + tname());
+    // dynamic LDS: the largest configuration present
+    int lds = 0;
+    for (int c : cfg) lds = std::max(lds, lds_bytes(c));
+    // grid <= 0: as many workgroups as the GPU holds at once (workgroups
+    // beyond that would find the queue empty or nearly so)
+    if (grid <= 0) chk(lt_hip_max_resident_blocks(fn, 256, (unsigned)lds, &grid), "occupancy");
     int g = std::max(1, std::min(grid, n_items));
-    void *a0 = d_descs, *a1 = d_items;
+    void *a0 = d_descs, *a1 = d_items, *a3 = d_queue;
     int a2 = n_items;
-    void *args[] = {&a0, &a1, &a2};
-    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, 0, stream, args), "grouped launch");
+    void *args[] = {&a0, &a1, &a2, &a3};
+    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, (unsigned)lds, stream, args), "grouped launch");
   }
 
   void plain(int cfg_all, void *stream) {
@@ -165,6 +184,7 @@ struct lth_batch {
     if (graph) lt_hip_graph_destroy(graph);
     if (d_descs) lt_hip_free(d_descs);
     if (d_items) lt_hip_free(d_items);
+    if (d_queue) lt_hip_free(d_queue);
   }
 };
 

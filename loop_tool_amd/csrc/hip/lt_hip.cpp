@@ -140,6 +140,21 @@ int lt_hip_launch(void *function, unsigned gx, unsigned gy, unsigned gz, unsigne
                "hipModuleLaunchKernel");
 }
 
+int lt_hip_max_resident_blocks(void *function, unsigned block, unsigned shared_bytes,
+                               int *blocks) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (int rc = check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &per_cu, (hipFunction_t)function, (int)block, shared_bytes),
+                     "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor"))
+    return rc;
+  if (int rc = check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+  if (int rc = check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                     "hipDeviceGetAttribute"))
+    return rc;
+  *blocks = per_cu * cus;
+  return 0;
+}
+
 int lt_hip_malloc(void **ptr, size_t bytes) {
   return check(hipMalloc(ptr, bytes ? bytes : 1), "hipMalloc");
 }

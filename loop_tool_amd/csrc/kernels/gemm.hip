@@ -1,16 +1,22 @@
 // Kernel library for batches of run-time-sized matmul nests (gfx950).
 //
 //   lt_gemm_<t>_<BM>x<BN>x<BK>(A, Bt, C, M, N, K)   one GEMM per launch
-//   lt_ggemm_<t>(descs, items, n_items)              persistent grouped launch
+//   lt_ggemm[_mid|_small]_<t>(descs, items, n_items, queue)
+//                                                    persistent grouped launch
 //   lt_cvt_<t>(src, dst, rows, cols, transpose)      float32 -> 16-bit operands
 //
 // Both launch forms run the software-pipelined implicit-GEMM tile of
 // igemm_tile.h (with its K-tail masking: K is any multiple of 8) on operands
 // A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
-// kernel walks a host-built work list of output tiles from many GEMMs
-// (ordered by decreasing cost); workgroup w handles items w, w + grid,
-// w + 2 grid, ...  Every item carries its own tile configuration, chosen per
-// shape bucket by the autotuner, so one launch mixes tile shapes.
+// kernel walks a host-built work list of output tiles from many GEMMs,
+// ordered by decreasing cost, as a queue: workgroup w starts with item w and
+// then takes item grid + queue[0]++ (fetched while it computes the previous
+// one) until the list is empty, so the longest tiles start first and the
+// workgroups finish together.  The last workgroup
+// to leave (counted in queue[1]) resets both counters for the next launch,
+// so graph replays need no memset; launches of one work list must not
+// overlap.  Every item carries its own tile configuration, chosen per shape
+// bucket by the autotuner, so one launch mixes tile shapes.
 #include <hip/hip_runtime.h>
 
 #include "igemm_tile.h"
@@ -54,11 +60,16 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
   return g;
 }
 
-// tile configurations: index -> (BM, BN, BK); LDS of the grouped kernel is
-// the largest of them (128x128x64: 73,728 B, two workgroups per CU)
-#define LT_CFG_LIST(_) \
-  _(0, 64, 64, 32) _(1, 128, 64, 32) _(2, 64, 128, 32) _(3, 128, 128, 64) _(4, 256, 128, 32) \
-  _(5, 128, 256, 32)
+// tile configurations: index -> (BM, BN, BK), in three register classes.  A
+// grouped kernel holds the code of every configuration of its class, and its
+// register count is that of the largest one, which sets how many waves a
+// SIMD holds: small (up to 128x64 / 64x128: three), mid (adds 128x128x32:
+// two), all (adds 128x128x64, 256x128 and 128x256: one).  The launch takes
+// the smallest class that covers the batch; its LDS is dynamic, sized for
+// the largest configuration present.
+#define LT_CFG_SMALL(_) _(0, 64, 64, 32) _(1, 128, 64, 32) _(2, 64, 128, 32)
+#define LT_CFG_MID(_) LT_CFG_SMALL(_) _(6, 128, 128, 32)
+#define LT_CFG_LIST(_) LT_CFG_MID(_) _(3, 128, 128, 64) _(4, 256, 128, 32) _(5, 128, 256, 32)
 
 #define LT_PLAIN(T, TN, BM, BN, BK)                                                          \
   extern "C" __global__ void __launch_bounds__(256)                                         \
@@ -76,20 +87,37 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
         g, A, Bt, d.C, it.m0, it.n0, smem, (unsigned)d.M * d.K * 2, (unsigned)d.N * d.K * 2); \
     break;
 
-#define LT_GROUPED(T_, TN)                                                                   \
+#define LT_GROUPED(T_, TN, CLASS, LIST)                                                      \
   extern "C" __global__ void __launch_bounds__(256)                                         \
-      lt_ggemm_##TN(const LtGemmDesc* __restrict__ descs, const LtWorkItem* __restrict__ items, \
-                    int n_items) {                                                          \
+      lt_ggemm##CLASS##_##TN(const LtGemmDesc* __restrict__ descs,                          \
+                             const LtWorkItem* __restrict__ items, int n_items,             \
+                             int* __restrict__ queue) {                                     \
     typedef T_ T;                                                                           \
-    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<128, 128, 64>()]; \
-    for (int i = blockIdx.x; i < n_items; i += gridDim.x) {                                 \
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];                  \
+    __shared__ int next_item;                                                               \
+    int i = blockIdx.x; /* the first items by index, the rest from the queue */             \
+    while (i < n_items) {                                                                   \
+      /* take the next item now: the atomic's latency hides behind this one */              \
+      int nxt = 0;                                                                          \
+      if (threadIdx.x == 0) nxt = (int)gridDim.x + atomicAdd(queue, 1);                     \
       const LtWorkItem it = items[i];                                                       \
       const LtGemmDesc d = descs[it.gemm];                                                  \
       const T* A = (const T*)d.A;                                                           \
       const T* Bt = (const T*)d.Bt;                                                         \
       const LtMmGeo g = lt_mm_geo(d.M, d.N, d.K);                                           \
-      switch (it.cfg) { LT_CFG_LIST(LT_CASE) default: break; }                              \
-      __syncthreads(); /* LDS reuse by the next item */                                    \
+      switch (it.cfg) { LIST(LT_CASE) default: break; }                                     \
+      __syncthreads(); /* LDS reuse; every lane has read next_item */                       \
+      if (threadIdx.x == 0) next_item = nxt;                                                \
+      __syncthreads();                                                                      \
+      i = next_item;                                                                        \
+    }                                                                                       \
+    if (threadIdx.x == 0) {                                                                 \
+      __threadfence(); /* this workgroup's last take precedes its exit count */             \
+      if (atomicAdd(queue + 1, 1) == (int)gridDim.x - 1) {                                  \
+        __threadfence(); /* every workgroup has taken its last item */                      \
+        atomicExch(queue, 0);                                                               \
+        atomicExch(queue + 1, 0);                                                           \
+      }                                                                                     \
     }                                                                                       \
   }
 
@@ -99,12 +127,17 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
   LT_PLAIN(T, TN, 64, 128, 32)                                                             \
   LT_PLAIN(T, TN, 128, 128, 64)                                                            \
   LT_PLAIN(T, TN, 256, 128, 32)                                                            \
-  LT_PLAIN(T, TN, 128, 256, 32)
+  LT_PLAIN(T, TN, 128, 256, 32)                                                            \
+  LT_PLAIN(T, TN, 128, 128, 32)
 
 LT_PLAIN_ALL(__bf16, bf16)
 LT_PLAIN_ALL(_Float16, f16)
-LT_GROUPED(__bf16, bf16)
-LT_GROUPED(_Float16, f16)
+LT_GROUPED(__bf16, bf16, , LT_CFG_LIST)
+LT_GROUPED(__bf16, bf16, _mid, LT_CFG_MID)
+LT_GROUPED(__bf16, bf16, _small, LT_CFG_SMALL)
+LT_GROUPED(_Float16, f16, , LT_CFG_LIST)
+LT_GROUPED(_Float16, f16, _mid, LT_CFG_MID)
+LT_GROUPED(_Float16, f16, _small, LT_CFG_SMALL)
 
 // float32 [rows][cols] -> 16-bit [rows][cols], or [cols][rows] when transpose
 #define LT_CVT(T, TN)                                                                       \

@@ -38,23 +38,31 @@ def test_plain_every_tile_vs_f64(bat, hip_dev, dtype):
             assert e <= 1e-2, (cfg, d, e)
 
 
-def test_grouped_and_graph_match_plain(bat, hip_dev):
+@pytest.mark.parametrize("cfgs", [None, [0, 1, 2], [0, 1, 2, 6]], ids=["all", "small", "mid"])
+def test_grouped_and_graph_match_plain(bat, hip_dev, cfgs):
+    """Each register class of the grouped kernel (chosen by the batch's
+    largest configuration) matches the plain launches to the bit."""
     dims = bat.shapes(n=64, seed=7, lo=32, hi=1024)
     b = bat.Batch(dims, dtype="bf16", seed=3)
     rng = np.random.RandomState(0)
-    cfg = [int(c) for c in rng.randint(0, len(bat.TILES), len(dims))]
+    choice = cfgs if cfgs is not None else list(range(len(bat.TILES)))
+    cfg = [int(choice[c]) for c in rng.randint(0, len(choice), len(dims))]
     b.set_configs(cfg)
     b.run_plain(-1)
     hip_dev.synchronize()
     want = [b.result(i) for i in range(len(dims))]
     for c in b.C:
         c.fill_zero()
-    b.run_grouped(grid=97)
-    hip_dev.synchronize()
-    for i in range(len(dims)):
-        assert np.array_equal(b.result(i), want[i]), ("grouped", i, dims[i], cfg[i])
-    for c in b.C:
-        c.fill_zero()
+    # the work queue resets itself after each launch: launches in a row, with
+    # fewer workgroups than the GPU holds, as many, and more than there are
+    # items (the surplus finds the queue empty)
+    for grid in (97, 0, 100000, 0):
+        b.run_grouped(grid=grid)
+        hip_dev.synchronize()
+        for i in range(len(dims)):
+            assert np.array_equal(b.result(i), want[i]), ("grouped", grid, i, dims[i], cfg[i])
+        for c in b.C:
+            c.fill_zero()
     b.capture(grid=256)
     for rep in range(3):
         b.replay()
