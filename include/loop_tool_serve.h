@@ -47,6 +47,8 @@ typedef struct {
   int keep_outputs;    /* keep outputs in HBM until lts_release */
   int checksums;       /* checksum every output on the device */
   int dtype;           /* contraction operands: 0 bf16, 1 fp16 */
+  int compiled_cap;    /* programs of no known family kept compiled, least
+                          recently used dropped first (0: 256) */
 } lts_config;
 
 typedef struct {
@@ -78,6 +80,7 @@ typedef struct {
   double host_retire_ms;         /* host time: completion polling and retirement */
   int64_t device_allocs;         /* device allocations made (blocks are reused) */
   double cached_bytes;           /* idle device bytes held for reuse */
+  int64_t compiled_evictions;    /* compiled programs dropped from the cache */
 } lts_stats;
 
 typedef struct {

@@ -291,14 +291,21 @@ void HipCompiled::load() const {
     hip_check(lt_hip_module_get_function(module_, s.kernel.c_str(), &fn), "get function");
     functions_.push_back(fn);
   }
+  hip_check(lt_hip_stream_create(&stream_), "stream");
+  loaded_ = true;
+}
+
+// The Compiled's own scratch, allocated on first use by run()/run_stage():
+// run_on() callers (the serving runtime) bring theirs, so a program compiled
+// only for them holds no device memory beyond its code.
+void HipCompiled::ensure_scratch() const {
+  if (!scratch_.empty()) return;
   for (const auto &b : plan_.buffers) {
     if (b.kind != BufferSpec::SCRATCH) continue;
     void *p = nullptr;
     hip_check(lt_hip_malloc(&p, b.numel * sizeof(float)), "scratch allocation");
     scratch_.push_back(p);
   }
-  hip_check(lt_hip_stream_create(&stream_), "stream");
-  loaded_ = true;
 }
 
 void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream, int first,
@@ -317,6 +324,7 @@ void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream
 }
 
 std::vector<void *> HipCompiled::device_buffers(const std::vector<void *> &io) const {
+  ensure_scratch();
   std::vector<void *> bufs;
   size_t scratch_i = 0;
   for (const auto &b : plan_.buffers) {

@@ -52,6 +52,7 @@ class HipCompiled : public Compiled {
 
  private:
   void load() const;
+  void ensure_scratch() const;
   void launch(const std::vector<void *> &device_buffers, void *stream, int first = 0,
               int last = -1) const;
   std::vector<void *> device_buffers(const std::vector<void *> &io) const;

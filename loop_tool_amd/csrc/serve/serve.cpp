@@ -29,6 +29,7 @@
 #include <cstring>
 #include <deque>
 #include <fstream>
+#include <list>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -501,7 +502,13 @@ struct Server {
   std::vector<int> retired;  // completed since the last take_retired()
   // caches
   std::unordered_map<uint64_t, int> family_of;  // structural key -> family
-  std::unordered_map<uint64_t, std::shared_ptr<Compiled>> compiled;  // exact lazy hash
+  // exact lazy hash -> compiled program, least recently used last (the
+  // eviction order of the reference's JS CompilationCache LRU,
+  // javascript/loop_tool.mjs:68-130); plans in flight keep their own
+  // reference, so an evicted program lives until its requests retire
+  std::list<uint64_t> lru;
+  std::unordered_map<uint64_t, std::pair<std::shared_ptr<Compiled>, std::list<uint64_t>::iterator>>
+      compiled;
   // stats
   lts_stats st{};
   std::vector<double> lat;
@@ -516,6 +523,7 @@ struct Server {
   explicit Server(const lts_config &c) : cfg(c) {
     ASSERT(cfg.slots >= 1 && cfg.slots <= 64) << "slots must be in [1, 64]";
     ASSERT(cfg.dtype == 0 || cfg.dtype == 1) << "dtype must be 0 (bf16) or 1 (fp16)";
+    ASSERT(cfg.compiled_cap >= 0) << "compiled_cap must be >= 0";
     kernels();
     template_keys();
     slots.resize(cfg.slots);
@@ -584,12 +592,20 @@ struct Server {
     auto it = compiled.find(h);
     if (it != compiled.end()) {
       ++st.exact_hits;
-      p->compiled = it->second;
+      p->compiled = it->second.first;
+      lru.splice(lru.begin(), lru, it->second.second);
     } else {
       ++st.compiles;
       auto lt = t.loop_tree();
       p->compiled = getBackends().at("hip")->compile(lt);
-      compiled[h] = p->compiled;
+      lru.push_front(h);
+      compiled[h] = {p->compiled, lru.begin()};
+      const size_t cap = cfg.compiled_cap > 0 ? (size_t)cfg.compiled_cap : 256;
+      while (compiled.size() > cap) {
+        compiled.erase(lru.back());
+        lru.pop_back();
+        ++st.compiled_evictions;
+      }
     }
     auto *hc = dynamic_cast<const hip::HipCompiled *>(p->compiled.get());
     ASSERT(hc) << "hip backend expected";

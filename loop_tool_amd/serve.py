@@ -25,7 +25,7 @@ FAMILIES = ["matmul", "softmax", "conv2d", "maxpool2d", "generic"]
 
 class Config(C.Structure):
     _fields_ = [("budget_bytes", C.c_double), ("slots", C.c_int), ("keep_outputs", C.c_int),
-                ("checksums", C.c_int), ("dtype", C.c_int)]
+                ("checksums", C.c_int), ("dtype", C.c_int), ("compiled_cap", C.c_int)]
 
 
 class RequestInfo(C.Structure):
@@ -52,7 +52,8 @@ class Stats(C.Structure):
                 ("peak_bytes", C.c_double), ("admission_waits", C.c_int64),
                 ("client_lag_ms", C.c_double), ("host_plan_ms", C.c_double),
                 ("host_launch_ms", C.c_double), ("host_retire_ms", C.c_double),
-                ("device_allocs", C.c_int64), ("cached_bytes", C.c_double)]
+                ("device_allocs", C.c_int64), ("cached_bytes", C.c_double),
+                ("compiled_evictions", C.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -109,9 +110,9 @@ def template_hash(family):
 
 class Server:
     def __init__(self, budget_gb=200.0, slots=8, keep_outputs=False, checksums=False,
-                 dtype="bf16"):
+                 dtype="bf16", compiled_cap=0):
         cfg = Config(budget_gb * 1e9, slots, int(keep_outputs), int(checksums),
-                     0 if dtype == "bf16" else 1)
+                     0 if dtype == "bf16" else 1, compiled_cap)
         self.h = _L.lts_create(C.byref(cfg))
         if not self.h:
             raise ServeError(_L.lts_last_error().decode())

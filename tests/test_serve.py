@@ -145,6 +145,35 @@ def test_unknown_structure_served_by_hip_backend(lt, server):
             server.release(rid)
 
 
+@pytest.mark.gpu
+def test_compiled_cache_is_bounded(lt, hip_dev):
+    """Programs of no family stay compiled up to `compiled_cap`, least
+    recently used dropped first; a dropped program compiles again on its
+    next request and still gives the oracle's result."""
+    from loop_tool_amd import serve as S
+    s = S.Server(budget_gb=1, slots=2, keep_outputs=True, compiled_cap=2)
+    progs = [P.pad1d(lt, 100 + 10 * i, 1, 2) for i in range(3)]
+
+    def run(k):
+        out, shapes = progs[k]
+        rid = s.submit(out, [7], [1.0])
+        s.drain()
+        x = host_inputs(shapes, [7], [1.0])
+        ref = np.zeros(out.loop_tree.io_numel()[1], np.float32)
+        lt.compile(out.loop_tree, "cpp").run(x + [ref])
+        assert np.array_equal(s.output(rid), ref)
+        s.release(rid)
+    for k in (0, 1, 0, 2):  # 1 is the least recently used when 2 arrives
+        run(k)
+    st = s.stats()
+    assert st["compiles"] == 3 and st["exact_hits"] == 1 and st["compiled_evictions"] == 1
+    run(0)  # still cached
+    run(1)  # evicted: compiled again
+    st = s.stats()
+    assert st["compiles"] == 4 and st["exact_hits"] == 2 and st["compiled_evictions"] == 2
+    s.close()
+
+
 def mixed_programs(lt, n=36, seed=5):
     rng = np.random.RandomState(seed)
     progs = []
