@@ -111,3 +111,14 @@ def test_bad_knob_values_are_errors(lt, knob, msg):
     with pytest.raises(lt_mod.LoopToolError, match=msg):
         lt.compile(t.annotate(t.roots[0], knob), "hip")
     lt.compile(t.annotate(t.roots[0], "parallel unroll=4 tile=64x64"), "hip")
+
+
+@pytest.mark.parametrize("name,kw", [("matmul", dict(M=0, N=4, K=4)),
+                                     ("softmax", dict(R=0, Cn=8))])
+def test_empty_extent_is_an_error(lt, name, kw):
+    """A zero-sized loop var is refused when the tree is built for a
+    backend, with the variable named (no kernel with an empty grid)."""
+    import loop_tool_amd as lt_mod
+    out, _ = getattr(P, name)(lt, **kw)
+    with pytest.raises(lt_mod.LoopToolError, match="empty extent"):
+        lt.compile(out.loop_tree, "hip")

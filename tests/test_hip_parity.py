@@ -366,3 +366,34 @@ def test_random_knob_sets(lt, hip_dev, name, kw, backend, knobs):
     ops = inputs if backend == "hip" else [_bf16_round(a) for a in inputs]
     f64 = P.reference_output(name, ops, **ref_kw(name, kw)).ravel()
     assert normwise(y, f64) <= 1e-5, (cc.string_property("kinds"), cc.dump().splitlines()[0])
+
+
+DEGENERATE = [
+    ("matmul", dict(M=1, N=1, K=1)), ("matmul", dict(M=1, N=300, K=1)),
+    ("softmax", dict(R=1, Cn=1)), ("softmax", dict(R=3, Cn=1)),
+    ("conv2d", dict(N=1, Ci=1, Co=1, H=1, W=1)),
+    ("conv2d", dict(N=1, Ci=3, Co=2, H=2, W=2, pad=0, k=2)),
+    ("maxpool2d", dict(N=1, C=1, H=2, W=2)), ("concat2d", dict(B=1, M0=1, M1=1)),
+    ("transpose2d", dict(M=1, N=1)), ("bmm", dict(Bn=1, M=1, N=1, K=1)),
+]
+
+
+@pytest.mark.parametrize("name,kw", DEGENERATE, ids=["%s%d" % (c[0], i) for i, c in enumerate(DEGENERATE)])
+@pytest.mark.parametrize("backend", ["hip", "hip_bf16"])
+def test_size_one_dims(lt, hip_dev, name, kw, backend):
+    """Size-1 dims everywhere (1x1 convs and images, single rows and
+    columns): the float path equals the oracle within 1e-5 (bit for bit
+    where lowered pointwise); 16-bit contractions match float64 of the
+    rounded operands."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = [P.normal(40 + i, s) for i, s in enumerate(shapes)]
+    y_or, _ = run_backend(lt, out, shapes, inputs, "cpp")
+    y, cc = run_backend(lt, out, shapes, inputs, backend)
+    kinds = cc.string_property("kinds")
+    if kinds.endswith("contract_bf16"):
+        f64 = P.reference_output(name, [_bf16_round(a) for a in inputs], **ref_kw(name, kw)).ravel()
+        assert normwise(y, f64) <= 1e-5, kinds
+    elif all(k == "pointwise" for k in kinds.split(",")):
+        assert np.array_equal(y.view(np.uint32), y_or.view(np.uint32)), kinds
+    else:
+        assert normwise(y, y_or) <= 1e-5, kinds
