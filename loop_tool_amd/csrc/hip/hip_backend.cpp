@@ -45,7 +45,9 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <list>
 #include <mutex>
 #include <sstream>
 #include <unordered_map>
@@ -92,10 +94,16 @@ uint64_t fnv1a(const std::string &s) {
   return h;
 }
 
-// In-process and on-disk cache of code objects, keyed by module source.
+// In-process and on-disk cache of code objects, keyed by module source.  The
+// in-process map keeps the most recently used kMemEntries; a disk entry
+// stores its source beside the code and is used only when the source
+// matches (the file name is a hash of it).
 struct CodeCache {
+  static constexpr size_t kMemEntries = 512;
+  using Code = std::shared_ptr<std::vector<char>>;
   std::mutex mu;
-  std::unordered_map<std::string, std::shared_ptr<std::vector<char>>> mem;
+  std::list<std::string> lru;  // sources, most recently used first
+  std::unordered_map<std::string, std::pair<Code, std::list<std::string>::iterator>> mem;
   std::string dir;
   CodeCache() {
     if (const char *d = std::getenv("LT_HIP_CACHE_DIR")) {
@@ -103,9 +111,23 @@ struct CodeCache {
     } else if (const char *h = std::getenv("HOME")) {
       dir = std::string(h) + "/.cache/loop_tool_amd";
     }
-    if (!dir.empty()) {
-      std::string cmd = "mkdir -p '" + dir + "' 2>/dev/null";
-      if (std::system(cmd.c_str()) != 0) dir.clear();
+    std::error_code ec;
+    if (!dir.empty() && !std::filesystem::create_directories(dir, ec) && ec) dir.clear();
+  }
+  // callers hold mu
+  Code find(const std::string &source) {
+    auto it = mem.find(source);
+    if (it == mem.end()) return nullptr;
+    lru.splice(lru.begin(), lru, it->second.second);
+    return it->second.first;
+  }
+  void put(const std::string &source, Code code) {
+    if (find(source)) return;
+    lru.push_front(source);
+    mem[source] = {std::move(code), lru.begin()};
+    while (mem.size() > kMemEntries) {
+      mem.erase(lru.back());
+      lru.pop_back();
     }
   }
 };
@@ -131,26 +153,30 @@ std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
   auto &cc = code_cache();
   {
     std::lock_guard<std::mutex> g(cc.mu);
-    auto it = cc.mem.find(source);
-    if (it != cc.mem.end()) {
+    if (auto code = cc.find(source)) {
       if (from_cache) *from_cache = true;
-      return it->second;
+      return code;
     }
   }
+  // disk entry: [source size: 8 bytes][source][code object]
   std::string path;
   if (!cc.dir.empty()) {
     std::stringstream p;
     p << cc.dir << "/" << std::hex << fnv1a(source) << "_" << source.size() << ".co";
     path = p.str();
     std::ifstream f(path, std::ios::binary);
-    if (f) {
-      auto buf = std::make_shared<std::vector<char>>(
-          (std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-      if (!buf->empty()) {
-        std::lock_guard<std::mutex> g(cc.mu);
-        cc.mem[source] = buf;
-        if (from_cache) *from_cache = true;
-        return buf;
+    uint64_t n = 0;
+    if (f && f.read((char *)&n, sizeof(n)) && n == source.size()) {
+      std::string stored(n, '\0');
+      if (f.read(&stored[0], (std::streamsize)n) && stored == source) {
+        auto buf = std::make_shared<std::vector<char>>((std::istreambuf_iterator<char>(f)),
+                                                       std::istreambuf_iterator<char>());
+        if (!buf->empty()) {
+          std::lock_guard<std::mutex> g(cc.mu);
+          cc.put(source, buf);
+          if (from_cache) *from_cache = true;
+          return buf;
+        }
       }
     }
   }
@@ -164,12 +190,15 @@ std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
     std::string tmp = path + ".tmp" + std::to_string((uint64_t)(uintptr_t)buf.get());
     {
       std::ofstream f(tmp, std::ios::binary);
+      const uint64_t n = source.size();
+      f.write((const char *)&n, sizeof(n));
+      f.write(source.data(), (std::streamsize)n);
       f.write(buf->data(), (std::streamsize)buf->size());
     }
     std::rename(tmp.c_str(), path.c_str());
   }
   std::lock_guard<std::mutex> g(cc.mu);
-  cc.mem[source] = buf;
+  cc.put(source, buf);
   if (from_cache) *from_cache = false;
   return buf;
 }

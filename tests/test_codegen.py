@@ -122,3 +122,29 @@ def test_empty_extent_is_an_error(lt, name, kw):
     out, _ = getattr(P, name)(lt, **kw)
     with pytest.raises(lt_mod.LoopToolError, match="empty extent"):
         lt.compile(out.loop_tree, "hip")
+
+
+def test_code_object_disk_cache(tmp_path):
+    """A second process loads the code object that the first compiled; an
+    entry whose stored source differs from the module's is not used."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import loop_tool_amd as lt\n"
+            "from loop_tool_amd import programs as P\n"
+            "out, _ = P.softmax(lt.lib, 64, 1000)\n"
+            "print(lt.lib.compile(out.loop_tree, 'hip').int_property('code_cached'))\n" % root)
+    env = dict(os.environ, LT_HIP_CACHE_DIR=str(tmp_path))
+
+    def cached():
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return int(r.stdout.strip().splitlines()[-1])
+    assert cached() == 0 and cached() == 1
+    (entry,) = list(tmp_path.iterdir())
+    b = entry.read_bytes()
+    entry.write_bytes(b[:8] + bytes([b[8] ^ 1]) + b[9:])  # one source byte changed
+    assert cached() == 0
