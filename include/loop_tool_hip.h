@@ -62,7 +62,9 @@ int lth_batch_num_items(lth_batch *b);
 /* one launch per GEMM with tile config `cfg` (-1: each GEMM's own config) */
 int lth_batch_run_plain(lth_batch *b, int cfg, void *stream);
 /* one persistent launch of `grid` workgroups over every tile of the batch
- * (grid <= 0: as many as the GPU holds at once for the batch's kernel) */
+ * (grid <= 0: as many as the GPU holds at once for the batch's kernel).  The
+ * tiles are handed out through a counter owned by the batch, so grouped
+ * launches (and graph replays) of one batch must not run concurrently. */
 int lth_batch_run_grouped(lth_batch *b, int grid, void *stream);
 /* capture the grouped launch into a hipGraph on `stream`, replay it later */
 int lth_batch_capture(lth_batch *b, int grid, void *stream);
