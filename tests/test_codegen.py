@@ -148,3 +148,14 @@ def test_code_object_disk_cache(tmp_path):
     b = entry.read_bytes()
     entry.write_bytes(b[:8] + bytes([b[8] ^ 1]) + b[9:])  # one source byte changed
     assert cached() == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N above 1 exits with a message."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
