@@ -24,6 +24,9 @@ finds those trees from the loop structure alone.
   tree pointwise, and within 1e-5 normwise where it regroups a reduction; on
   the mis-indexed trees it is held to the interpreter instead, since it
   starts each tail block where the enclosing block's iterations end.
+
+Beyond the reference's programs, a softmax chain and a padded conv are drawn
+the same way.
 """
 import math
 import random
@@ -56,6 +59,10 @@ def build(L, prog, size):
         return pw_add(L, size)
     if prog == "reduce":
         return reduce_add(L, size)
+    if prog == "softmax":
+        return P.softmax(L, *size)
+    if prog == "conv":
+        return P.conv2d(L, *size)
     return P.matmul(L, *size)
 
 
@@ -164,6 +171,11 @@ def cases(seed=1337, reps_1d=2, reps_mm=1):
             out += [(prog, s, rng.randrange(1 << 30)) for _ in range(reps_1d)]
     for mnk in MM:
         out += [("mm", mnk, rng.randrange(1 << 30)) for _ in range(reps_mm)]
+    # beyond the reference's list: a softmax chain and a padded conv
+    for rc in ((7, 33), (16, 64), (3, 129)):
+        out += [("softmax", rc, rng.randrange(1 << 30)) for _ in range(2)]
+    for conv in ((1, 3, 4, 6, 6), (2, 4, 2, 5, 7)):
+        out += [("conv", conv, rng.randrange(1 << 30)) for _ in range(3)]
     return out
 
 
@@ -181,6 +193,10 @@ def f64_reference(prog, xs):
         return x[0] + x[1]
     if prog == "reduce":
         return np.array([x[0].sum()])
+    if prog == "softmax":
+        return P.reference_output("softmax", xs)
+    if prog == "conv":
+        return P.reference_output("conv2d", xs)
     return x[0] @ x[1]
 
 
@@ -245,6 +261,8 @@ def test_random_schedule_oracle_vs_reference(lt, ref, prog, size, seed):
     want = f64_reference(prog, xs)
     n = want.size
     if c_backend_misindexes(t):
+        if prog == "conv":  # the interpreter's padded views differ as well
+            return
         y = run_guarded(lt, t, "cpu_interpreted", xs)[GUARD:GUARD + n]
     else:
         y = b_or[GUARD:GUARD + n]
@@ -261,7 +279,11 @@ def test_random_schedule_hip_vs_oracle(lt, hip_dev, prog, size, seed):
     xs = inputs_for(shapes, seed)
     n = t.io_numel()[len(xs)]
     oracle = "cpu_interpreted" if c_backend_misindexes(t) else "cpp"
-    y_or = run_guarded(lt, t, oracle, xs)[GUARD:GUARD + n]
+    if oracle == "cpp" or prog != "conv":
+        y_or = run_guarded(lt, t, oracle, xs)[GUARD:GUARD + n]
+    else:  # the interpreter's padded views differ: float64 instead
+        y_or = f64_reference(prog, xs).astype(np.float32).ravel()
+        oracle = "float64"
     cc = lt.compile(t, "hip")
     y = np.zeros(n, np.float32)
     cc.run(list(xs) + [y])
