@@ -141,26 +141,41 @@ struct lth_batch {
     }
   }
 
-  void grouped(int grid, void *stream) {
-    upload();
-    if (!n_items) return;
-    // the smallest register class of grouped kernel covering the batch's
-    // configurations (gemm.hip: small 0-2, mid adds 6, the full kernel all)
+  // the grouped kernel for the batch's configurations: the smallest register
+  // class covering them (gemm.hip: small 0-2, mid adds 6, the full kernel
+  // all), and its dynamic LDS, sized for the largest configuration present
+  void *grouped_kernel(unsigned *lds) {
     int cls = 0;
-    for (int c : cfg) cls = std::max(cls, c <= 2 ? 0 : c == 6 ? 1 : 2);
+    *lds = 0;
+    for (int c : cfg) {
+      cls = std::max(cls, c <= 2 ? 0 : c == 6 ? 1 : 2);
+      *lds = std::max(*lds, (unsigned)lds_bytes(c));
+    }
     const char *kname[3] = {"lt_ggemm_small_", "lt_ggemm_mid_", "lt_ggemm_"};
-    void *fn = gemm_module().fn(std::string(kname[cls]) + tname());
-    // dynamic LDS: the largest configuration present
-    int lds = 0;
-    for (int c : cfg) lds = std::max(lds, lds_bytes(c));
-    // grid <= 0: as many workgroups as the GPU holds at once (workgroups
-    // beyond that would find the queue empty or nearly so)
-    if (grid <= 0) chk(lt_hip_max_resident_blocks(fn, 256, (unsigned)lds, &grid), "occupancy");
+    return gemm_module().fn(std::string(kname[cls]) + tname());
+  }
+
+  // grid <= 0: as many workgroups as the GPU holds at once (workgroups
+  // beyond that would find the queue empty or nearly so)
+  int resolve_grid(int grid) {
+    upload();
+    if (grid > 0) return grid;
+    unsigned lds = 0;
+    void *fn = grouped_kernel(&lds);
+    chk(lt_hip_max_resident_blocks(fn, 256, lds, &grid), "occupancy");
+    return grid;
+  }
+
+  void grouped(int grid, void *stream) {
+    grid = resolve_grid(grid);
+    if (!n_items) return;
+    unsigned lds = 0;
+    void *fn = grouped_kernel(&lds);
     int g = std::max(1, std::min(grid, n_items));
     void *a0 = d_descs, *a1 = d_items, *a3 = d_queue;
     int a2 = n_items;
     void *args[] = {&a0, &a1, &a2, &a3};
-    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, (unsigned)lds, stream, args), "grouped launch");
+    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, lds, stream, args), "grouped launch");
   }
 
   void plain(int cfg_all, void *stream) {
@@ -272,8 +287,8 @@ int lth_batch_run_grouped(lth_batch *b, int grid, void *stream) {
 int lth_batch_capture(lth_batch *b, int grid, void *stream) {
   return guarded([&] {
     ASSERT(stream) << "graph capture needs a created stream (not the null stream)";
-    b->upload();
-    gemm_module();  // load outside capture
+    // uploads, module load and the occupancy query happen outside capture
+    grid = b->resolve_grid(grid);
     if (b->graph) {
       lt_hip_graph_destroy(b->graph);
       b->graph = nullptr;
