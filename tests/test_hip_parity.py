@@ -397,3 +397,24 @@ def test_size_one_dims(lt, hip_dev, name, kw, backend):
         assert np.array_equal(y.view(np.uint32), y_or.view(np.uint32)), kinds
     else:
         assert normwise(y, y_or) <= 1e-5, kinds
+
+
+def test_run_without_sync_on_device_buffers(lt, hip_dev):
+    """Compiled::run(memory, sync=false) returns after enqueueing
+    (backend.h:18-22): several runs queued back to back on device buffers
+    give the oracle's result once the device is synchronised; with host
+    buffers the call stages through device memory and returns complete."""
+    out, shapes = P.softmax(lt, 512, 4096)
+    x = P.normal(3, shapes[0], 3.0)
+    y_or = np.zeros(512 * 4096, np.float32)
+    lt.compile(out.loop_tree, "cpp").run([x, y_or])
+    cc = lt.compile(out.loop_tree, "hip")
+    dx, dy = hip_dev.DeviceArray.from_numpy(x), hip_dev.DeviceArray(512 * 4096)
+    for _ in range(4):
+        cc.run([dx.ptr, dy.ptr], sync=False)
+    hip_dev.synchronize()
+    y = dy.download()
+    assert normwise(y, y_or) <= 1e-5
+    yh = np.zeros_like(y_or)
+    cc.run([x, yh], sync=False)
+    assert np.array_equal(yh.view(np.uint32), y.view(np.uint32))
