@@ -47,6 +47,23 @@ def parse():
     return ap.parse_args()
 
 
+SETTLE_S = 0.2  # untimed GPU work before the warmup steps
+
+
+def settle(step, sync, seconds=SETTLE_S, max_steps=5000):
+    """Run `step` untimed for about `seconds` (at most `max_steps` times):
+    the GPU's clocks ramp up over the first milliseconds of work after idle,
+    and a short timed region measured before that reads 4-20 % slow.
+    Returns the number of steps run."""
+    n, t0 = 0, time.perf_counter()
+    while n < max_steps and time.perf_counter() - t0 < seconds:
+        for _ in range(10):
+            step()
+        n += 10
+        sync()
+    return n
+
+
 def workload(lt, name):
     from loop_tool_amd import programs as P
     if name == "softmax":
@@ -204,6 +221,7 @@ def bench_batched(args):
         grouped_ms = B.time_ms(lambda: b.run_grouped(), iters=10, stream=stream.h)
         b.capture()
         if dt == "bf16":
+            settle_steps = settle(b.replay, stream.synchronize)
             for _ in range(args.warmup):
                 b.replay()
             stream.synchronize()
@@ -240,6 +258,7 @@ def bench_batched(args):
         "config": dict(workload="512 independent matmul nests, M,N,K log-uniform in "
                                 "[32, 2048] (multiples of 16), bf16 operands, fp32 accumulate",
                        gemms=len(dims), algorithmic_flops=flops, algorithmic_bytes=nbytes,
+                       settle_steps=settle_steps, settle_s=SETTLE_S,
                        bf16=out["bf16"], f16=out["f16"],
                        note="the batch's arithmetic intensity (flops/bytes) is below the "
                             "MI355X ridge point (2.5 PFLOP/s / 8 TB/s), so HBM bounds it"),
@@ -394,6 +413,8 @@ def main():
         ref = P.reference_output(name, inputs_np).ravel()
         err = float(np.abs(y - ref).max() / np.abs(ref).max())
 
+    ptrs = [b.ptr for b in io]
+    settle_steps = settle(lambda: cc.run(ptrs, sync=False), hip.synchronize)
     for _ in range(args.warmup):
         cc.run([b.ptr for b in io], sync=False)
     hip.synchronize()
@@ -455,7 +476,8 @@ def main():
                        algorithmic_bytes=bytes_step, algorithmic_flops=flops_step,
                        dominant_kernel=st[dom]["kernel"], kernel_ms=round(kern_ms, 5),
                        gpu_ms_per_step=round(gpu_step_ms, 5),
-                       plan_ms=round(plan_ms, 5), max_err_vs_f64=err),
+                       plan_ms=round(plan_ms, 5), max_err_vs_f64=err,
+                       settle_steps=settle_steps, settle_s=SETTLE_S),
         "roofline": roof,
     }
     if not args.no_cpu:
