@@ -1,6 +1,6 @@
 """loop_tool_amd: loop_tool's frontend and an MI355X (gfx950) "hip" backend.
 
-The native library (liblt_amd.so, built by `python -m loop_tool_amd.build`)
+The native library (liblt_amd.so, built by `python loop_tool_amd/build.py`)
 holds the C++ loop_tool core, lazy frontend, schedule transforms and the hip
 backend; this package binds its C ABI (include/loop_tool_c.h) with ctypes and
 exposes the API of the reference's `loop_tool_py` module
@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "liblt_amd.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(
         "loop_tool_amd native library missing at %s; build it with "
-        "`python -m loop_tool_amd.build`" % LIB_PATH)
+        "`python loop_tool_amd/build.py`" % LIB_PATH)
 
 lib = capi.Library(LIB_PATH)
 

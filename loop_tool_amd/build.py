@@ -1,7 +1,7 @@
 """Build the native libraries of loop_tool_amd in-tree.
 
-    python -m loop_tool_amd.build            # product library only
-    python -m loop_tool_amd.build --all      # + the test oracle libraries
+    python loop_tool_amd/build.py            # product library only
+    python loop_tool_amd/build.py --all      # + the test oracle libraries
 
 Outputs (git-ignored, shipped with the tree):
     loop_tool_amd/liblt_amd.so     C++ loop_tool core, lazy frontend, schedule

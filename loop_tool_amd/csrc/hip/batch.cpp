@@ -81,7 +81,7 @@ GemmModule &gemm_module() {
     loop_tool::hip::ensure_runtime();
     std::string path = library_dir() + "/kernels/gemm.hsaco";
     std::ifstream f(path, std::ios::binary);
-    ASSERT(f.is_open()) << "kernel library missing: " << path << " (python -m loop_tool_amd.build)";
+    ASSERT(f.is_open()) << "kernel library missing: " << path << " (python loop_tool_amd/build.py)";
     auto *mm = new GemmModule();
     mm->code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
     chk(lt_hip_module_load(mm->code.data(), &mm->mod), "load gemm.hsaco");

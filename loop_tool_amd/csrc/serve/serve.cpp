@@ -90,7 +90,7 @@ Kernels &kernels() {
     hip::ensure_runtime();
     std::string path = library_dir() + "/kernels/serve.hsaco";
     std::ifstream f(path, std::ios::binary);
-    ASSERT(f.is_open()) << "kernel library missing: " << path << " (python -m loop_tool_amd.build)";
+    ASSERT(f.is_open()) << "kernel library missing: " << path << " (python loop_tool_amd/build.py)";
     auto *kk = new Kernels();
     kk->code.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
     chk(lt_hip_module_load(kk->code.data(), &kk->mod), "load serve.hsaco");

@@ -159,3 +159,19 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
+
+
+def test_build_script_runs_without_the_package():
+    """The build script named in the package's error messages runs as a
+    plain script, so it works in a tree whose native library is not built
+    yet (importing the package needs that library)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "loop_tool_amd", "build.py"),
+                        "--help"], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, PYTHONPATH=""))
+    assert r.returncode == 0 and "--all" in r.stdout
+    src = open(os.path.join(root, "loop_tool_amd", "build.py")).read()
+    assert "import loop_tool_amd" not in src and "from loop_tool_amd" not in src
