@@ -509,6 +509,10 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
     if (ld) load(kt + 2, RF);
     const bool next_halo = kb + 1 < KB;
     if (tap == 0 && next_halo) hload(kb + 1);
+    // raised wave priority while this wave issues its MFMAs: the sequencer
+    // favours it over waves that are loading or storing (measured 3-4 %
+    // faster on the 3x3 conv tile)
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int kk = 0; kk < KK; ++kk) {
       const int s = kk & 1;
@@ -519,6 +523,7 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
 #pragma unroll
         for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
     }
+    __builtin_amdgcn_s_setprio(0);
     if (HB == 2 && tap == NTAP - 1 && next_halo) hstore((kb + 1) & 1);
     if (st) {
       __syncthreads();
