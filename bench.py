@@ -38,12 +38,14 @@ def parse():
     ap.add_argument("--nest", default="softmax")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--duration", type=float, default=10.0,
-                    help="serving: seconds of Poisson arrivals (the full workload is 60)")
+    ap.add_argument("--duration", type=float, default=60.0,
+                    help="serving: seconds of Poisson arrivals")
     ap.add_argument("--rate", type=float, default=2000.0, help="serving: arrivals per second")
     ap.add_argument("--slots", type=int, default=8, help="serving: requests in flight")
     ap.add_argument("--annotate", default="",
                     help="hip backend knobs for the nest, e.g. 'tile=128x256'")
+    ap.add_argument("--no-conv", action="store_true",
+                    help="softmax run: skip the conv2d bf16 kernel and plan figures")
     return ap.parse_args()
 
 
@@ -111,6 +113,71 @@ def workload(lt, name):
         return out, shapes, gen, dict(workload="2x2 max-pool stride 2", N=32, C=128,
                                       H=112, W=112), "hbm", "f32", "hip"
     raise SystemExit("unknown nest " + name)
+
+
+# error bars per nest: error vs float64 of the float operands ("f64"), and of
+# the 16-bit-rounded operands for 16-bit contractions ("f64_rounded")
+TOLERANCE = {"softmax": {"f64": 1e-5}, "matmul": {"f64": 1e-5}, "matmul_f32": {"f64": 1e-5},
+             "matmul_bf16": {"f64": 1e-2, "f64_rounded": 1e-5},
+             "conv2d": {"f64": 1e-5}, "conv2d_bf16": {"f64": 1e-2, "f64_rounded": 1e-5},
+             "maxpool": {"f64": 0.0}, "copy": {"f64": 0.0}}
+
+
+def nest_errors(nest, inputs_np, y):
+    """Normwise error of the device result y against float64 closed forms:
+    every row of the softmax; every element of matmul 128, max-pool and copy;
+    128 rows of large matmuls; for convolutions, every border pixel of two
+    batch planes of every output channel plus 65,536 random outputs."""
+    from loop_tool_amd import programs as P
+
+    def nw(a, b):
+        return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+    kind = "matmul_bf16" if nest.startswith("matmul_bf16") else (
+        "matmul_f32" if nest.startswith("matmul_f32") else nest)
+    out = {}
+    if nest == "softmax":
+        x = inputs_np[0]
+        R, C = x.shape
+        yy = y.reshape(R, C)
+        worst, scale = 0.0, 0.0
+        for r0 in range(0, R, 1024):
+            ref = P.reference_output("softmax", [x[r0:r0 + 1024]])
+            worst = max(worst, float(np.abs(yy[r0:r0 + 1024] - ref).max()))
+            scale = max(scale, float(np.abs(ref).max()))
+        out["f64"] = worst / scale
+        out["rows_checked"] = R
+    elif nest == "copy":
+        out["f64"] = float(np.abs(y - inputs_np[0].ravel()).max())
+    elif nest == "maxpool":
+        ref = P.reference_output("maxpool2d", inputs_np).ravel()
+        out["f64"] = float(np.abs(y - ref).max())
+    elif nest == "matmul":
+        out["f64"] = nw(y, P.reference_output("matmul", inputs_np).ravel())
+    elif kind in ("matmul_f32", "matmul_bf16"):
+        a, b = inputs_np
+        M = a.shape[0]
+        rows = np.unique(np.concatenate([np.arange(64), np.random.RandomState(1).randint(0, M, 64)]))
+        yy = y.reshape(M, b.shape[1])[rows]
+        out["f64"] = nw(yy, a[rows].astype(np.float64) @ b.astype(np.float64))
+        if kind == "matmul_bf16":
+            out["f64_rounded"] = nw(yy, P.round_bf16(a[rows]).astype(np.float64)
+                                    @ P.round_bf16(b).astype(np.float64))
+        out["rows_checked"] = int(rows.size)
+    elif nest in ("conv2d", "conv2d_bf16"):
+        X, W = inputs_np
+        N, Co, H = X.shape[0], W.shape[0], X.shape[2]
+        idx = P.conv_sample_indices(N, Co, H, H)
+        got = y.reshape(N, Co, H, H)[idx[:, 0], idx[:, 1], idx[:, 2], idx[:, 3]]
+        out["f64"] = nw(got, P.conv_f64_at(X, W, idx))
+        if nest == "conv2d_bf16":
+            out["f64_rounded"] = nw(got, P.conv_f64_at(P.round_bf16(X), P.round_bf16(W), idx))
+        out["outputs_checked"] = int(len(idx))
+        out["sample"] = "every border pixel of batch planes 0 and N-1 of every output " \
+                        "channel, plus 65,536 uniformly drawn outputs"
+    tol = TOLERANCE.get(kind, {})
+    out["ok"] = all(out[k] <= t for k, t in tol.items() if k in out)
+    out["tolerance"] = tol
+    return out
 
 
 def cpu_baseline(name, inputs_np, shapes, seconds, bytes_per_step, flops_per_step):
@@ -215,7 +282,6 @@ def bench_batched(args):
         b = B.Batch(dims, dtype=dt, seed=6, stream=stream.h)
         b.run_plain(B.DEFAULT_CFG)
         stream.synchronize()
-        errs = [b.check(i) for i in (0, 1, 2, 3)]
         plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
         cfg, report = B.tune_buckets(b, iters=3)
         grouped_ms = B.time_ms(lambda: b.run_grouped(), iters=10, stream=stream.h)
@@ -232,13 +298,14 @@ def bench_batched(args):
             el = time.perf_counter() - t0
         # the dominant (only) kernel of a replay: HIP events on its stream
         graph_ms = B.time_ms(b.replay, iters=max(args.steps, 20), stream=stream.h)
-        errs += [b.check(i) for i in (0, 1, 2, 3)]
+        e, e16 = batch_errors(b)
         out[dt] = dict(plain_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
                        graph_ms=round(graph_ms, 5),
                        speedup_vs_plain=round(plain_ms / graph_ms, 3),
                        work_items=b.num_items(), tuning=report,
-                       max_err_vs_f64=max(e for e, _ in errs),
-                       max_err_vs_f64_of_rounded_operands=max(e for _, e in errs))
+                       max_err_vs_f64=e, max_err_vs_f64_of_rounded_operands=e16,
+                       error_sample="48 random rows (all rows when fewer) of every GEMM, "
+                                    "after the graph replays")
         b.free()
     kern_ms = out["bf16"]["graph_ms"]
     achieved = nbytes / (kern_ms * 1e-3) / 1e9
@@ -268,6 +335,30 @@ def bench_batched(args):
     if not args.no_cpu:
         line["cpu_baseline"] = batched_cpu_baseline(dims, args.cpu_seconds)
     print(json.dumps(line))
+    bad = [dt for dt in out if out[dt]["max_err_vs_f64"] > 1e-2
+           or out[dt]["max_err_vs_f64_of_rounded_operands"] > 1e-5]
+    if bad:
+        print("bench.py: batched result outside its error bar: %s" % bad, file=sys.stderr)
+        sys.exit(3)
+
+
+def batch_errors(b, rows_per_gemm=48, seed=13):
+    """(worst normwise error vs float64 of the float operands, worst vs
+    float64 of the 16-bit-rounded operands) over sampled rows of every GEMM
+    of the batch."""
+    from loop_tool_amd import programs as P
+    rnd = P.round_bf16 if b.dtype == 0 else (lambda a: a.astype(np.float16).astype(np.float32))
+    rng = np.random.RandomState(seed)
+    worst = worst16 = 0.0
+    for i, (m, n, k) in enumerate(b.dims):
+        a, bb = b.host[i]
+        rows = np.sort(rng.choice(m, min(m, rows_per_gemm), replace=False))
+        got = b.result(i)[rows].astype(np.float64)
+        ref = a[rows].astype(np.float64) @ bb.astype(np.float64)
+        ref16 = rnd(a[rows]).astype(np.float64) @ rnd(bb).astype(np.float64)
+        worst = max(worst, float(np.abs(got - ref).max() / np.abs(ref).max()))
+        worst16 = max(worst16, float(np.abs(got - ref16).max() / np.abs(ref16).max()))
+    return worst, worst16
 
 
 def serving_cpu_baseline(seconds, seed=7):
@@ -395,23 +486,10 @@ def main():
     bytes_step = sum(s["bytes"] for s in st)
     flops_step = sum(s["flops"] for s in st)
 
-    # correctness spot check against float64 on the device result
-    from loop_tool_amd import programs as P
+    # correctness against float64 on the device result
     cc.run([b.ptr for b in io])
     hip.synchronize()
-    y = d_out.download()
-    if args.nest == "softmax":
-        ref = P.reference_output("softmax", [inputs_np[0][:64]]).ravel()
-        err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
-    elif args.nest == "copy":
-        err = float(np.abs(y - inputs_np[0].ravel()).max())
-    elif args.nest.startswith("matmul_bf16") or args.nest.startswith("matmul_f32"):
-        ref = P.reference_output("matmul", [inputs_np[0][:64], inputs_np[1]]).ravel()
-        err = float(np.abs(y[:ref.size] - ref).max() / np.abs(ref).max())
-    else:
-        name = {"conv2d_bf16": "conv2d", "maxpool": "maxpool2d"}.get(args.nest, args.nest)
-        ref = P.reference_output(name, inputs_np).ravel()
-        err = float(np.abs(y - ref).max() / np.abs(ref).max())
+    errors = nest_errors(args.nest, inputs_np, d_out.download())
 
     ptrs = [b.ptr for b in io]
     settle_steps = settle(lambda: cc.run(ptrs, sync=False), hip.synchronize)
@@ -450,6 +528,7 @@ def main():
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
                     traffic=MEASURED_TRAFFIC.get(args.nest))
         metric = "HBM GB/s (algorithmic bytes / time), hip backend"
+        plan_frac = bytes_step / (plan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
     else:
         value = flops_step * args.steps / el / 1e12
         unit = "TFLOP/s"
@@ -458,6 +537,7 @@ def main():
         roof = dict(bound="mfma", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=None)
         metric = "TFLOP/s (algorithmic flops / time), hip backend"
+        plan_frac = flops_step / (plan_ms * 1e-3) / 1e12 / peak
 
     line = {
         "metric": metric,
@@ -476,15 +556,58 @@ def main():
                        algorithmic_bytes=bytes_step, algorithmic_flops=flops_step,
                        dominant_kernel=st[dom]["kernel"], kernel_ms=round(kern_ms, 5),
                        gpu_ms_per_step=round(gpu_step_ms, 5),
-                       plan_ms=round(plan_ms, 5), max_err_vs_f64=err,
+                       plan_ms=round(plan_ms, 5), plan_frac=round(plan_frac, 4),
+                       max_err_vs_f64=errors.get("f64"), errors=errors,
                        settle_steps=settle_steps, settle_s=SETTLE_S),
         "roofline": roof,
     }
+    if args.nest == "softmax" and not args.no_conv:
+        line["config"]["conv2d_bf16"] = conv_bf16_figures(L, hip)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.nest, inputs_np, shapes, args.cpu_seconds,
                                             bytes_step if bound == "hbm" else 0,
                                             flops_step if bound != "hbm" else 0)
     print(json.dumps(line))
+    bad = [k for k, v in [("", errors)] + [
+        ("conv2d_bf16", line["config"].get("conv2d_bf16", {}).get("errors", {"ok": True}))]
+        if not v["ok"]]
+    if bad:
+        print("bench.py: result outside its error bar: %s" % (bad,), file=sys.stderr)
+        sys.exit(3)
+
+
+def conv_bf16_figures(L, hip):
+    """The BF16 convolution N=32, C 128 -> 128, 56 x 56, 3 x 3 pad 1 measured
+    in the same run as the softmax: the MFMA kernel's and the whole plan's
+    (staging + kernel) fraction of the 2.5 PFLOP/s bf16 peak, each the median
+    of three HIP-event timings of 20 launches, and the border-inclusive
+    sampled error against float64."""
+    out, shapes, gen, cfg, _, _, backend = workload(L, "conv2d_bf16")
+    cc = L.compile(out.loop_tree, backend)
+    inputs_np = gen()
+    d_in = [hip.DeviceArray.from_numpy(a) for a in inputs_np]
+    d_out = hip.DeviceArray(out.loop_tree.io_numel()[2])
+    io = d_in + [d_out]
+    st = hip.stages(cc)
+    flops = sum(s["flops"] for s in st)
+    dom = max(range(len(st)), key=lambda i: st[i]["flops"])
+    cc.run([b.ptr for b in io])
+    hip.synchronize()
+    errors = nest_errors("conv2d_bf16", inputs_np, d_out.download())
+    ptrs = [b.ptr for b in io]
+    settle(lambda: cc.run(ptrs, sync=False), hip.synchronize)
+    kern_ms = sorted(hip.time_stage_ms(cc, io, dom, warmup=3, iters=20) for _ in range(3))[1]
+    plan_ms = sorted(hip.time_ms(cc, io, warmup=3, iters=20) for _ in range(3))[1]
+    peak = MFMA_PEAK_TFLOPS["bf16"]
+    kt = st[dom]["flops"] / (kern_ms * 1e-3) / 1e12
+    pt = flops / (plan_ms * 1e-3) / 1e12
+    return dict(workload=cfg["workload"] + ", bf16 operands, fp32 accumulate",
+                kernels=[s["kernel"] + ":" + s["kind"] for s in st],
+                plan=cc.dump().splitlines()[0], algorithmic_flops=flops,
+                kernel_ms=round(kern_ms, 5), kernel_tflops=round(kt, 2),
+                kernel_frac=round(kt / peak, 4), plan_ms=round(plan_ms, 5),
+                plan_tflops=round(pt, 2), plan_frac=round(pt / peak, 4), peak_tflops=peak,
+                errors=errors)
 
 
 if __name__ == "__main__":

@@ -105,6 +105,10 @@ int ltc_tensor_compile(const ltc_tensor *t);
 ltc_looptree *ltc_tensor_loop_tree(const ltc_tensor *t);
 int ltc_tensor_set_loop_tree(const ltc_tensor *t, const ltc_looptree *lt);
 ltc_compiled *ltc_tensor_compiled(const ltc_tensor *t);
+/* drop the program's compiled and lowered cache entries (python.cpp:566) */
+int ltc_tensor_clear_cache(const ltc_tensor *t);
+/* the next data access recomputes the tensor (python.cpp:565) */
+int ltc_tensor_force_recompute(const ltc_tensor *t);
 
 /* LoopTree: python.cpp:228-337 */
 void ltc_looptree_free(ltc_looptree *lt);

@@ -84,6 +84,8 @@ class Library:
         s("ltc_tensor_loop_tree", _P, _P)
         s("ltc_tensor_set_loop_tree", C.c_int, _P, _P)
         s("ltc_tensor_compiled", _P, _P)
+        s("ltc_tensor_clear_cache", C.c_int, _P)
+        s("ltc_tensor_force_recompute", C.c_int, _P)
         s("ltc_looptree_free", None, _P)
         s("ltc_looptree_dump", C.c_int, _P, C.c_char_p, C.c_int)
         s("ltc_looptree_size", C.c_int, _P)
@@ -192,12 +194,12 @@ class Library:
                 self.lib.ltc_compile(loop_tree.h, backend.encode())))
 
     def convolve(self, X, W, spatial, window, stride=1):
-        return Tensor(self, self.check_ptr(self.lib.ltc_nn_convolve(
-            X.h, W.h, _handles(spatial), _handles(window), len(spatial), stride)))
+        return _derived(self, self.lib.ltc_nn_convolve(
+            X.h, W.h, _handles(spatial), _handles(window), len(spatial), stride), X, W)
 
     def maxpool(self, X, spatial, k, stride=1):
-        return Tensor(self, self.check_ptr(self.lib.ltc_nn_maxpool(
-            X.h, _handles(spatial), len(spatial), k, stride)))
+        return _derived(self, self.lib.ltc_nn_maxpool(
+            X.h, _handles(spatial), len(spatial), k, stride), X)
 
 
 def _handles(objs):
@@ -308,6 +310,17 @@ class Expr(_Handle):
         return self.L.string(self.L.lib.ltc_expr_dump, self.h)
 
 
+def _derived(L, h, *srcs):
+    """A tensor computed from `srcs`: it keeps their bound host arrays alive,
+    since the native graph holds only their addresses."""
+    t = Tensor(L, L.check_ptr(h))
+    keep = []
+    for s in srcs:
+        keep += getattr(s, "_keepalive", [])
+    t._keepalive = keep
+    return t
+
+
 class Tensor(_Handle):
     _free = "ltc_tensor_free"
 
@@ -318,7 +331,7 @@ class Tensor(_Handle):
             sizes = (C.c_int64 * len(arr.shape))(*arr.shape)
             t = Tensor(lib, lib.check_ptr(lib.lib.ltc_tensor_from_data(
                 arr.ctypes.data, sizes, len(arr.shape))))
-            t._keep = arr
+            t._keepalive = [arr]
             return t
         if args and all(isinstance(a, Symbol) for a in args):
             return Tensor(lib, lib.check_ptr(lib.lib.ltc_tensor_from_symbols(
@@ -328,10 +341,10 @@ class Tensor(_Handle):
         return Tensor(lib, lib.check_ptr(lib.lib.ltc_tensor_from_sizes(arr, len(sizes))))
 
     def _bin(self, op, o):
-        return Tensor(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_binary(op, self.h, o.h)))
+        return _derived(self.L, self.L.lib.ltc_tensor_binary(op, self.h, o.h), self, o)
 
     def _un(self, op):
-        return Tensor(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_unary(op, self.h)))
+        return _derived(self.L, self.L.lib.ltc_tensor_unary(op, self.h), self)
 
     def __add__(self, o):
         return self._bin(0, o)
@@ -373,8 +386,8 @@ class Tensor(_Handle):
         return self._un(5)
 
     def _red(self, op, syms):
-        return Tensor(self.L, self.L.check_ptr(
-            self.L.lib.ltc_tensor_reduce(op, self.h, _handles(syms), len(syms))))
+        return _derived(self.L, self.L.lib.ltc_tensor_reduce(
+            op, self.h, _handles(syms), len(syms)), self)
 
     def sum(self, *syms):
         return self._red(0, syms)
@@ -389,32 +402,31 @@ class Tensor(_Handle):
         return self._red(3, syms)
 
     def as_(self, *syms):
-        return Tensor(self.L, self.L.check_ptr(
-            self.L.lib.ltc_tensor_as(self.h, _handles(syms), len(syms))))
+        return _derived(self.L, self.L.lib.ltc_tensor_as(self.h, _handles(syms), len(syms)),
+                        self)
 
     def to(self, syms, constraints=()):
         lhs = [Expr.of(self.L, c[0]) for c in constraints]
         rhs = [Expr.of(self.L, c[1]) for c in constraints]
-        return Tensor(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_to(
-            self.h, _handles(syms), len(syms), _handles(lhs), _handles(rhs), len(lhs))))
+        return _derived(self.L, self.L.lib.ltc_tensor_to(
+            self.h, _handles(syms), len(syms), _handles(lhs), _handles(rhs), len(lhs)), self)
 
     def pad(self, sym, pre, post=None):
         post = pre if post is None else post
-        return Tensor(self.L, self.L.check_ptr(
-            self.L.lib.ltc_tensor_pad(self.h, sym.h, pre, post)))
+        return _derived(self.L, self.L.lib.ltc_tensor_pad(self.h, sym.h, pre, post), self)
 
     def transpose(self, *syms):
-        return Tensor(self.L, self.L.check_ptr(
-            self.L.lib.ltc_tensor_transpose(self.h, _handles(syms), len(syms))))
+        return _derived(self.L, self.L.lib.ltc_tensor_transpose(
+            self.h, _handles(syms), len(syms)), self)
 
     def flatten(self, syms, new_sym):
-        return Tensor(self.L, self.L.check_ptr(self.L.lib.ltc_tensor_flatten(
-            self.h, _handles(syms), len(syms), new_sym.h)))
+        return _derived(self.L, self.L.lib.ltc_tensor_flatten(
+            self.h, _handles(syms), len(syms), new_sym.h), self)
 
     def bind(self, data, sizes=None):
         if data is not None:
-            self._keep = data
-            sizes = data.shape
+            self._keepalive = getattr(self, "_keepalive", []) + [data]
+            sizes = data.shape if sizes is None else sizes
         arr = (C.c_int64 * len(sizes))(*sizes)
         ptr = data.ctypes.data if data is not None else None
         self.L.check_int(self.L.lib.ltc_tensor_bind(self.h, ptr, arr, len(sizes)))
@@ -449,9 +461,29 @@ class Tensor(_Handle):
         n = int(np.prod(sizes)) if sizes else 1
         return np.ctypeslib.as_array(p, shape=(n,)).reshape(sizes).copy()
 
+    def numpy_into(self, out):
+        """Evaluate into a caller array, as the reference's `numpy()` does
+        (python.cpp:548-561): the array is bound as the tensor's output
+        buffer, then the program runs on the default backend."""
+        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+        sizes = self.sizes
+        assert out.size == (int(np.prod(sizes)) if sizes else 1)
+        self.bind(out, sizes)
+        with self.L.compiling():
+            self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
+        return out.reshape(sizes)
+
     def resolve(self):
         with self.L.compiling():
             self.L.check_ptr(self.L.lib.ltc_tensor_data(self.h))
+
+    def clear_cache(self):
+        """Drop this program's compiled and lowered cache entries
+        (python.cpp:566)."""
+        self.L.check_int(self.L.lib.ltc_tensor_clear_cache(self.h))
+
+    def force_recompute(self):
+        self.L.check_int(self.L.lib.ltc_tensor_force_recompute(self.h))
 
     @property
     def code(self):

@@ -332,6 +332,20 @@ ltc_compiled *ltc_tensor_compiled(const ltc_tensor *t) {
   return guard_ptr([&] { return new ltc_compiled{t->t.compiled()}; });
 }
 
+int ltc_tensor_clear_cache(const ltc_tensor *t) {
+  return guard_int([&] {
+    t->t.clear_cache();
+    return 0;
+  });
+}
+
+int ltc_tensor_force_recompute(const ltc_tensor *t) {
+  return guard_int([&] {
+    t->t.force_recompute();
+    return 0;
+  });
+}
+
 // --------------------------------------------------------------- LoopTree ----
 void ltc_looptree_free(ltc_looptree *lt) { delete lt; }
 

@@ -98,8 +98,22 @@ TensorImpl::TensorImpl(const TensorImpl &o)
   }
 }
 
+// An owned buffer goes back to the hardware that allocated it: the highest
+// id in its compatibility mask (CPU memory is 0x1; a device's host-visible
+// memory carries 0x1 | 1 << id).  The reference frees through whatever
+// hardware is the default at destruction (lazy.h:120-123), which hands such
+// memory to the wrong allocator once the default has changed.
 TensorImpl::~TensorImpl() {
-  if (owning_ && memory_.address) getDefaultHardware()->free(memory_);
+  if (!owning_ || !memory_.address) return;
+  int id = 0;
+  for (int b = 0; b < 32; ++b)
+    if (memory_.compatible & (1 << b)) id = b;
+  for (const auto &hw : getHardware())
+    if (hw->id() == id) {
+      hw->free(memory_);
+      return;
+    }
+  getDefaultHardware()->free(memory_);
 }
 
 std::unordered_map<int, Expr> TensorImpl::size_constraints() const {

@@ -45,41 +45,52 @@ def normal(seed, shape, std=1.0):
 
 
 # ---------------------------------------------------------------- programs --
-def matmul(L, M, N, K):
+# Every builder takes `data`: None for size-only leaves (the program is then
+# compiled and run explicitly on buffers), or one float32 array per input,
+# bound to the leaves so that the lazy frontend can evaluate the program
+# itself (`Tensor.numpy()`).
+def _leaf(L, shape, data, i):
+    if data is None:
+        return L.Tensor(*shape)
+    a = np.ascontiguousarray(data[i], np.float32).reshape(shape)
+    return L.Tensor(data=a)
+
+
+def matmul(L, M, N, K, data=None):
     """C[m,n] = sum_k A[m,k] * B[k,n]."""
     m, n, k = L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
-    A = L.Tensor(M, K)
-    B = L.Tensor(K, N)
+    A = _leaf(L, (M, K), data, 0)
+    B = _leaf(L, (K, N), data, 1)
     C = (A.as_(m, k) * B.as_(k, n)).sum(k)
     return C, [(M, K), (K, N)]
 
 
-def bmm(L, Bn, M, N, K):
+def bmm(L, Bn, M, N, K, data=None):
     """C[b,m,n] = sum_k A[b,m,k] * B[b,k,n]: a batch of matmuls sharing b."""
     b, m, n, k = L.Symbol("B"), L.Symbol("M"), L.Symbol("N"), L.Symbol("K")
-    A = L.Tensor(Bn, M, K)
-    B = L.Tensor(Bn, K, N)
+    A = _leaf(L, (Bn, M, K), data, 0)
+    B = _leaf(L, (Bn, K, N), data, 1)
     C = (A.as_(b, m, k) * B.as_(b, k, n)).sum(k)
     return C, [(Bn, M, K), (Bn, K, N)]
 
 
-def softmax(L, R, Cn):
+def softmax(L, R, Cn, data=None):
     """Row softmax over C: max-reduce, subtract, exp, sum-reduce, divide."""
     r, c = L.Symbol("R"), L.Symbol("C")
-    X = L.Tensor(R, Cn).as_(r, c)
+    X = _leaf(L, (R, Cn), data, 0).as_(r, c)
     mx = X.max_reduce(c)
     e = (X - mx).exp()
     Y = e / e.sum(c)
     return Y, [(R, Cn)]
 
 
-def conv2d(L, N, Ci, Co, H, W, k=3, pad=1, stride=1):
+def conv2d(L, N, Ci, Co, H, W, k=3, pad=1, stride=1, data=None):
     """Y[n,co,ho,wo] = sum_{ci,kh,kw} Xpad[n,ci,ho*s+kh,wo*s+kw] * W[co,ci,kh,kw]
     with Xpad the zero-padded view of X (a pad nest plus a window sum)."""
     n, ci, h, w = L.Symbol("N"), L.Symbol("Ci"), L.Symbol("H"), L.Symbol("W")
     co, kh, kw = L.Symbol("Co"), L.Symbol("Kh"), L.Symbol("Kw")
-    X = L.Tensor(N, Ci, H, W).as_(n, ci, h, w)
-    Wt = L.Tensor(Co, Ci, k, k).as_(co, ci, kh, kw)
+    X = _leaf(L, (N, Ci, H, W), data, 0).as_(n, ci, h, w)
+    Wt = _leaf(L, (Co, Ci, k, k), data, 1).as_(co, ci, kh, kw)
     if pad:
         X = X.pad(h, pad).pad(w, pad)
         sh, sw = X.shape[2], X.shape[3]
@@ -92,36 +103,87 @@ def conv2d(L, N, Ci, Co, H, W, k=3, pad=1, stride=1):
     return Y, [(N, Ci, H, W), (Co, Ci, k, k)]
 
 
-def maxpool2d(L, N, C, H, W, k=2, stride=2):
+def maxpool2d(L, N, C, H, W, k=2, stride=2, data=None):
     n, c, h, w = L.Symbol("N"), L.Symbol("C"), L.Symbol("H"), L.Symbol("W")
-    X = L.Tensor(N, C, H, W).as_(n, c, h, w)
+    X = _leaf(L, (N, C, H, W), data, 0).as_(n, c, h, w)
     return L.maxpool(X, [h, w], k, stride), [(N, C, H, W)]
 
 
-def pad1d(L, N, pre, post):
+def pad1d(L, N, pre, post, data=None):
     n = L.Symbol("N")
-    X = L.Tensor(N).as_(n)
+    X = _leaf(L, (N,), data, 0).as_(n)
     return X.pad(n, pre, post), [(N,)]
 
 
-def concat2d(L, B, M0, M1):
+def pad2d(L, N, C, H, W, pad=1, data=None):
+    """Zero padding of the two spatial dims of an NCHW tensor (the pad nest
+    of a convolution's input, materialised)."""
+    n, c, h, w = L.Symbol("N"), L.Symbol("C"), L.Symbol("H"), L.Symbol("W")
+    X = _leaf(L, (N, C, H, W), data, 0).as_(n, c, h, w)
+    return X.pad(h, pad).pad(w, pad), [(N, C, H, W)]
+
+
+def concat2d(L, B, M0, M1, data=None):
     b, m0, m1 = L.Symbol("N"), L.Symbol("M0"), L.Symbol("M1")
-    A = L.Tensor(B, M0).as_(b, m0)
-    Bt = L.Tensor(B, M1).as_(b, m1)
+    A = _leaf(L, (B, M0), data, 0).as_(b, m0)
+    Bt = _leaf(L, (B, M1), data, 1).as_(b, m1)
     return A | Bt, [(B, M0), (B, M1)]
 
 
-def transpose2d(L, M, N):
+def transpose2d(L, M, N, data=None):
     m, n = L.Symbol("M"), L.Symbol("N")
-    A = L.Tensor(M, N).as_(m, n)
+    A = _leaf(L, (M, N), data, 0).as_(m, n)
     return A.to([n, m]), [(M, N)]
 
 
-def add_broadcast(L, M, N):
+def add_broadcast(L, M, N, data=None):
     m, n = L.Symbol("M"), L.Symbol("N")
-    A = L.Tensor(M, N).as_(m, n)
-    b = L.Tensor(N).as_(n)
+    A = _leaf(L, (M, N), data, 0).as_(m, n)
+    b = _leaf(L, (N,), data, 1).as_(n)
     return A + b, [(M, N), (N,)]
+
+
+def conv_sample_indices(N, Co, H, W, n_random=65536, seed=12):
+    """(n, co, h, w) output indices for a sampled float64 check of a
+    convolution: every border pixel of the planes n in {0, N-1} for every
+    output channel, plus n_random outputs drawn uniformly."""
+    hh, ww = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    border = (hh == 0) | (hh == H - 1) | (ww == 0) | (ww == W - 1)
+    bh, bw = hh[border], ww[border]
+    idx = []
+    for n in sorted({0, N - 1}):
+        for co in range(Co):
+            idx.append(np.stack([np.full_like(bh, n), np.full_like(bh, co), bh, bw], 1))
+    rng = np.random.RandomState(seed)
+    idx.append(np.stack([rng.randint(0, N, n_random), rng.randint(0, Co, n_random),
+                         rng.randint(0, H, n_random), rng.randint(0, W, n_random)], 1))
+    return np.concatenate(idx)
+
+
+def conv_f64_at(X, Wt, idx, pad=1, chunk=4096):
+    """float64 stride-1 convolution outputs Y[n, co, h, w] at the given
+    indices (closed form: a dot product over channels and window taps)."""
+    Xp = np.pad(X.astype(np.float64), ((0, 0), (0, 0), (pad, pad), (pad, pad)))
+    Wd = Wt.astype(np.float64)
+    k = Wt.shape[2]
+    ci = np.arange(X.shape[1])
+    out = np.empty(len(idx))
+    for s in range(0, len(idx), chunk):
+        n, co, h, w = idx[s:s + chunk].T
+        acc = np.zeros(len(n))
+        for a in range(k):
+            for b in range(k):
+                xs = Xp[n[:, None], ci[None, :], (h + a)[:, None], (w + b)[:, None]]
+                acc += np.einsum("sc,sc->s", xs, Wd[co, :, a, b])
+        out[s:s + chunk] = acc
+    return out
+
+
+def round_bf16(a):
+    """float32 -> nearest bf16 (ties to even), returned as float32."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
 
 
 def reference_output(name, inputs, **kw):
@@ -145,6 +207,13 @@ def reference_output(name, inputs, **kw):
                 patch = Xp[:, :, a:a + stride * Ho:stride, b:b + stride * Wo:stride]
                 Y += np.einsum("nchw,oc->nohw", patch, W[:, :, a, b])
         return Y
+    if name == "pad2d":
+        p = kw.get("pad", 1)
+        return np.pad(x[0], ((0, 0), (0, 0), (p, p), (p, p)))
+    if name == "concat2d":
+        return np.concatenate([x[0], x[1]], axis=1)
+    if name == "transpose2d":
+        return x[0].T
     if name == "maxpool2d":
         X = x[0]
         k, s = kw.get("k", 2), kw.get("stride", 2)

@@ -13,7 +13,12 @@ produced them on the repository's seeded inputs:
     reference rejected it.
 Programs with exp are pinned by tolerance only (libm versions differ).
 
-    python tests/golden/make_golden.py
+With --configs it writes tests/golden/config_programs.json instead: the
+benchmark configurations of tests/test_golden_configs.py (CONFIGS) at their
+full sizes -- LoopTree dump, flop count, and the SHA-256 of the reference C
+backend's output (strided output samples for the softmax, which uses exp).
+
+    python tests/golden/make_golden.py [--configs]
 """
 import hashlib
 import json
@@ -33,12 +38,38 @@ from test_oracle import PROGRAMS, norm_ids, run  # noqa: E402
 
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libloop_tool_ref.so")
 OUT = os.path.join(HERE, "oracle_programs.json")
+OUT_CONFIGS = os.path.join(HERE, "config_programs.json")
+
+
+def configs(ref):
+    import test_golden_configs as G
+    entries = []
+    for cid, name, kw, gen, exact in G.CONFIGS:
+        out, shapes = getattr(P, name)(ref, **kw)
+        inputs = gen(shapes)
+        tree = out.loop_tree
+        y = G.run_tree(ref, tree, inputs, "cpp")
+        e = dict(id=cid, program=name, kw=kw, dump=norm_ids(tree.dump()),
+                 flops=int(tree.flops()), numel=int(y.size), exact=exact)
+        if exact:
+            e["sha256"] = G.sha256(y)
+        else:
+            e["sample_stride"] = G.SAMPLE_STRIDE
+            e["sample"] = [float(v) for v in G.sample(y)]
+        entries.append(e)
+        print(cid, "done", flush=True)
+    with open(OUT_CONFIGS, "w") as f:
+        json.dump(dict(source="reference loop_tool built from its sources (oracle/_ref)",
+                       programs=entries), f, indent=1)
+    print("wrote %d configurations to %s" % (len(entries), OUT_CONFIGS))
 
 
 def main():
     if not os.path.exists(REF_LIB):
         raise SystemExit("build the reference library first: python -m oracle.build")
     ref = capi.Library(REF_LIB)
+    if "--configs" in sys.argv[1:]:
+        return configs(ref)
     entries = []
     for i, (name, kw, gen) in enumerate(PROGRAMS):
         out, shapes = getattr(P, name)(ref, **kw)

@@ -46,11 +46,35 @@ for name, kw in cases:
         cc.run(ins + [y_hip])
         L.compile(tree, "cpp").run(ins + [y_ref])
         rec["err"] = float(np.abs(y_hip - y_ref).max() / np.abs(y_ref).max())
-        # the reference lazy frontend with hip as its default backend
+        # the reference's own lazy frontend evaluating the program with hip as
+        # its default backend: data<float>() compiles through the plugin and
+        # runs it (include/loop_tool/lazy.h:219-260), with the output owned
+        # by the CPU hardware, by the plugin's pinned-memory hardware, or
+        # bound to a caller array (the reference Python numpy())
         L.set_default_backend("hip")
-        prog2, shapes2 = getattr(P, name)(L, **kw)
-        rec["lazy_equal"] = True
+        lazy = {}
+        for hw in ("cpu", "hip", "bound"):
+            L.set_default_hardware("cpu" if hw == "bound" else hw)
+            prog2, _ = getattr(P, name)(L, data=ins, **kw)
+            prog2.clear_cache()
+            if hw == "bound":
+                y_lazy = np.full(n, np.nan, np.float32)
+                prog2.numpy_into(y_lazy)
+            else:
+                y_lazy = prog2.numpy().ravel()
+            rec.setdefault("lazy_compiled", []).append(prog2.compiled.name)
+            prog2.clear_cache()
+            # the reference frees owned outputs through the default hardware
+            # (lazy.h:120-123): release it before the default changes back
+            del prog2
+            lazy[hw] = y_lazy
+        L.set_default_hardware("cpu")
         L.set_default_backend("cpp")
+        rec["lazy_err"] = {hw: float(np.abs(v - y_ref).max() / np.abs(y_ref).max())
+                           for hw, v in lazy.items()}
+        rec["lazy_bitwise_vs_compile"] = {
+            hw: bool(np.array_equal(v.view(np.uint32), y_hip.view(np.uint32)))
+            for hw, v in lazy.items()}
     out["cases"].append(rec)
 print("RESULT " + json.dumps(out))
 sys.stdout.flush()
@@ -85,3 +109,10 @@ def test_reference_library_runs_hip_plugin_on_gpu(hip_dev):
         print(c)
         assert c["compiled"] == "hip"
         assert c["err"] <= 1e-5, c
+        # the reference's lazy frontend drove the plugin, three ways
+        assert c["lazy_compiled"] == ["hip"] * 3, c
+        for hw, e in c["lazy_err"].items():
+            assert e <= 1e-5, (c["name"], hw, e)
+        # same plugin, same LoopTree, same inputs: the same bits as the
+        # direct Backend::compile run
+        assert all(c["lazy_bitwise_vs_compile"].values()), c
