@@ -46,6 +46,19 @@ int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int 
 int lth_time_stage(const ltc_compiled *cc, void *const *io, int n_io, int stage, int warmup,
                    int iters, float *ms);
 
+/* ---- allocation-free launch.  No counterpart in the reference: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  As lth_run_on, but the
+ * plan's scratch buffers come from one static device array of
+ * LTH_STATIC_SCRATCH_BYTES per GPU (held from the first call on that GPU
+ * until the process ends); no device memory is allocated or freed and
+ * nothing is copied or synchronised.  The Compiled's code must already be
+ * loaded (any earlier ltc_compiled_run, or lth_compiled_stream).  Plans
+ * needing more scratch are rejected (return -1, lth_last_error()).  Calls
+ * on one GPU share its array: calls that overlap on one GPU race. */
+#define LTH_STATIC_SCRATCH_BYTES (64u << 20)
+int lth_run_static(const ltc_compiled *cc, void *const *io, int n_io, void *stream);
+
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
  * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
  * transposed Bt[N][K]; tile configurations 0..5 are 64x64, 128x64, 64x128,

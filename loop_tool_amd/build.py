@@ -42,10 +42,13 @@ HIPCC_FLAGS = [
 
 
 def sources():
+    """Host C++ (.cpp, built with g++) and HIP (.hip, built with hipcc: the
+    static device scratch of the allocation-free launch) of the library."""
     out = []
     for sub in ("core", "frontends", "hip", "serve"):
         d = os.path.join(CSRC, sub)
-        out += [os.path.join(d, f) for f in sorted(os.listdir(d)) if f.endswith(".cpp")]
+        out += [os.path.join(d, f) for f in sorted(os.listdir(d))
+                if f.endswith(".cpp") or f.endswith(".hip")]
     out.append(os.path.join(CSRC, "capi.cpp"))
     out.append(os.path.join(CSRC, "capi_hip.cpp"))
     return out
@@ -88,11 +91,15 @@ def build_product(force=False, jobs=8):
     objs, todo = [], []
     for src in sources():
         rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
-        obj = os.path.join(BUILD, rel[:-4] + ".o")
+        obj = os.path.join(BUILD, rel.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
         stale = (force or not os.path.exists(obj)
                  or os.path.getmtime(obj) < max(os.path.getmtime(src), newest_header))
-        if stale:
+        if stale and src.endswith(".hip"):
+            flags = [f for f in CXXFLAGS if f != "-D__HIP_PLATFORM_AMD__"]
+            todo.append([os.path.join(ROCM, "bin", "hipcc")] + HIPCC_FLAGS + flags +
+                        ["-c", src, "-o", obj])
+        elif stale:
             todo.append(["g++"] + CXXFLAGS + ["-c", src, "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, todo))

@@ -42,6 +42,13 @@ int copy_str(const std::string &s, char *buf, int cap) {
 
 }  // namespace
 
+namespace loop_tool {
+namespace hip {
+// the message lth_last_error() reports, for entry points defined elsewhere
+void set_lth_error(const std::string &msg) { g_err = msg; }
+}  // namespace hip
+}  // namespace loop_tool
+
 extern "C" {
 
 const char *lth_last_error(void) { return g_err.c_str(); }

@@ -40,6 +40,13 @@ class HipCompiled : public Compiled {
   // then outputs) and the Compiled's own scratch (timing of one kernel).
   void run_stage(const std::vector<void *> &device_io, int k, void *stream) const;
 
+  // true once the plan's code object is loaded on the device (by any run,
+  // run_on, run_stage or stream())
+  bool loaded() const {
+    std::lock_guard<std::mutex> g(mu_);
+    return loaded_;
+  }
+
   const Plan &plan() const { return plan_; }
   int64_t algorithmic_bytes() const { return algorithmic_bytes_; }
   int64_t algorithmic_flops() const { return algorithmic_flops_; }
