@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Backend plugin interface of loop_tool: a Backend compiles a LoopTree into a
 // Compiled object whose run() takes one buffer per IR input and output, in
 // IR::inputs() then IR::outputs() order (dense row-major float32).

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Lowering analysis shared by every backend: variable sizes, per-loop inner
 // strides, buffer allocations (inputs | outputs | scratch) with their LCA
 // scope, and per-access index equations with bound predicates.

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Assertion helper of loop_tool: `ASSERT(cond) << "message"` throws a
 // std::runtime_error carrying the condition text, the source location and the
 // streamed message when `cond` is false.

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Hardware registry: CPU is hardware 0; every registered device gets the next
 // id.  An allocation reports the set of hardware that can address it as a
 // bitmask in Memory::compatible.

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Dataflow IR of loop_tool and the LoopTree derived from it.
 //
 // Interface kept from include/loop_tool/ir.h:19-422 of the reference: the

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Lazy symbolic tensor frontend of loop_tool (`lz::Tensor`, `Symbol`,
 // `.as/.to/.sum`).  A Tensor records an op graph over named symbols; data()
 // unifies sizes, lowers the graph to IR with the default schedule, builds the

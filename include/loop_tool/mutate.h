@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Schedule transforms: pure LoopTree -> LoopTree edits (split, merge, swap,
 // reuse control, annotation, unrolling) plus schedule analyses (flop count,
 // trivially-parallel loops).  They are the move set of the autotuner.

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Neural-network helpers built from lazy views: convolution as a window view
 // times weights summed over the window and shared channels, and max-pooling
 // as a window view max-reduced.  Interface of include/loop_tool/nn.h:14-19.

@@ -1,3 +1,12 @@
+/*
+Copyright (c) Facebook, Inc. and its affiliates.
+
+This source code is licensed under the MIT license found in the
+LICENSE file in the root directory of this source tree.
+
+The declarations below restate the reference loop_tool's public interface
+(include/loop_tool/) so that code written against it compiles unchanged.
+*/
 // Text serialization of an IR (nodes, constraints, orders, annotations,
 // reuse flags, inputs/outputs) so that a tuned schedule can be stored and
 // reloaded.  Same entry points as include/loop_tool/serialization.h of the

@@ -52,6 +52,8 @@
 #include <sstream>
 #include <unordered_map>
 
+#include <unistd.h>
+
 #include "emit.h"
 #include "loop_tool/backend.h"
 #include "loop_tool/hardware.h"
@@ -147,6 +149,29 @@ void ensure_runtime() {
   runtime_ready() = true;
 }
 
+namespace {
+std::string disk_path(const CodeCache &cc, const std::string &source) {
+  if (cc.dir.empty()) return "";
+  std::stringstream p;
+  p << cc.dir << "/" << std::hex << fnv1a(source) << "_" << source.size() << ".co";
+  return p.str();
+}
+}  // namespace
+
+void evict_code(const std::string &source) {
+  auto &cc = code_cache();
+  {
+    std::lock_guard<std::mutex> g(cc.mu);
+    auto it = cc.mem.find(source);
+    if (it != cc.mem.end()) {
+      cc.lru.erase(it->second.second);
+      cc.mem.erase(it);
+    }
+  }
+  auto path = disk_path(cc, source);
+  if (!path.empty()) std::remove(path.c_str());
+}
+
 std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
                                                   const std::string &name,
                                                   bool *from_cache) {
@@ -159,11 +184,8 @@ std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
     }
   }
   // disk entry: [source size: 8 bytes][source][code object]
-  std::string path;
-  if (!cc.dir.empty()) {
-    std::stringstream p;
-    p << cc.dir << "/" << std::hex << fnv1a(source) << "_" << source.size() << ".co";
-    path = p.str();
+  const std::string path = disk_path(cc, source);
+  if (!path.empty()) {
     std::ifstream f(path, std::ios::binary);
     uint64_t n = 0;
     if (f && f.read((char *)&n, sizeof(n)) && n == source.size()) {
@@ -187,15 +209,22 @@ std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
   auto buf = std::make_shared<std::vector<char>>((char *)code, (char *)code + size);
   lt_hip_free_code(code);
   if (!path.empty()) {
-    std::string tmp = path + ".tmp" + std::to_string((uint64_t)(uintptr_t)buf.get());
+    // written under a name of this process and call, published by rename
+    // only when every byte reached the file
+    static std::atomic<uint64_t> serial{0};
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid()) + "_" +
+                            std::to_string(serial++);
+    bool ok;
     {
       std::ofstream f(tmp, std::ios::binary);
       const uint64_t n = source.size();
       f.write((const char *)&n, sizeof(n));
       f.write(source.data(), (std::streamsize)n);
       f.write(buf->data(), (std::streamsize)buf->size());
+      f.close();
+      ok = f.good();
     }
-    std::rename(tmp.c_str(), path.c_str());
+    if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
   }
   std::lock_guard<std::mutex> g(cc.mu);
   cc.put(source, buf);
@@ -314,7 +343,15 @@ HipCompiled::~HipCompiled() {
 void HipCompiled::load() const {
   if (loaded_) return;
   ensure_runtime();
-  hip_check(lt_hip_module_load(code_->data(), &module_), "module load");
+  if (lt_hip_module_load(code_->data(), &module_) != 0) {
+    // a cached code object that does not load (a damaged cache file): drop
+    // it and compile the source again
+    if (!int_properties.count("code_cached") || !int_properties.at("code_cached"))
+      hip_check(1, "module load");
+    evict_code(plan_.source);
+    code_ = compile_source(plan_.source, "lt_module.hip", nullptr);
+    hip_check(lt_hip_module_load(code_->data(), &module_), "module load (recompiled)");
+  }
   for (const auto &s : plan_.stages) {
     void *fn = nullptr;
     hip_check(lt_hip_module_get_function(module_, s.kernel.c_str(), &fn), "get function");

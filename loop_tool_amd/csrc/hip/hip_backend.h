@@ -19,6 +19,8 @@ bool runtime_exiting();
 std::shared_ptr<std::vector<char>> compile_source(const std::string &source,
                                                   const std::string &name,
                                                   bool *from_cache);
+// drop a module source's code object from the in-process and disk caches
+void evict_code(const std::string &source);
 Knobs knobs_from(const LoopTree &lt, bool bf16_variant);
 
 class HipCompiled : public Compiled {
@@ -66,7 +68,7 @@ class HipCompiled : public Compiled {
 
   Plan plan_;
   Knobs knobs_;
-  std::shared_ptr<std::vector<char>> code_;
+  mutable std::shared_ptr<std::vector<char>> code_;
   int64_t algorithmic_bytes_ = 0;
   int64_t algorithmic_flops_ = 0;
 

@@ -148,6 +148,46 @@ def test_code_object_disk_cache(tmp_path):
     b = entry.read_bytes()
     entry.write_bytes(b[:8] + bytes([b[8] ^ 1]) + b[9:])  # one source byte changed
     assert cached() == 0
+    # the write is published by rename: no temporary file stays behind
+    assert [p.name for p in tmp_path.iterdir()] == [entry.name]
+
+
+@pytest.mark.gpu
+def test_damaged_disk_cache_entry_is_recompiled(tmp_path, hip_dev):
+    """A disk entry with the right source but a code object the runtime
+    rejects (not an ELF image) is dropped and the module compiled again:
+    the run succeeds and matches a run with a clean cache."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import numpy as np\n"
+            "import loop_tool_amd as lt\n"
+            "from loop_tool_amd import hip, programs as P\n"
+            "hip.init()\n"
+            "out, sh = P.softmax(lt.lib, 64, 1000)\n"
+            "cc = lt.lib.compile(out.loop_tree, 'hip')\n"
+            "y = np.zeros(64000, np.float32)\n"
+            "cc.run([P.normal(3, sh[0], 3.0), y])\n"
+            "print(cc.int_property('code_cached'), float(y.sum()))\n" % root)
+    env = dict(os.environ, LT_HIP_CACHE_DIR=str(tmp_path))
+
+    def run():
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        c, total = r.stdout.strip().splitlines()[-1].split()
+        return int(c), float(total)
+    c0, t0 = run()
+    assert c0 == 0
+    (entry,) = list(tmp_path.iterdir())
+    b = entry.read_bytes()
+    n = int.from_bytes(b[:8], "little")
+    entry.write_bytes(b[:8 + n] + b"not a code object")
+    c1, t1 = run()
+    assert c1 == 1 and t1 == t0
+    assert entry.read_bytes()[8 + n:8 + n + 4] == b"\x7fELF"  # rewritten
 
 
 def test_bench_refuses_more_than_one_gpu():
