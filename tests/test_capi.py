@@ -50,3 +50,18 @@ def test_errors_cross_the_boundary_as_codes(lt):
     s = lt.Symbol("X")
     with pytest.raises(lt_mod.LoopToolError):
         lt.Tensor(s).numpy()  # unbound size
+
+
+def test_batch_refuses_operands_of_2_gib():
+    """lth_batch_create refuses a GEMM whose 16-bit operand reaches 2^31
+    bytes (the tile's buffer ranges and row offsets are 32-bit)."""
+    lib = ctypes.CDLL(LIB)
+    lib.lth_batch_create.restype = ctypes.c_void_p
+    lib.lth_batch_last_error.restype = ctypes.c_char_p
+    arr = lambda v: (ctypes.c_int * 1)(v)
+    h = lib.lth_batch_create(1, arr(65536), arr(64), arr(16384), 0)
+    assert not h and b"2^31 bytes" in lib.lth_batch_last_error()
+    h = lib.lth_batch_create(1, arr(65535), arr(64), arr(16384), 0)
+    assert h
+    lib.lth_batch_free.argtypes = [ctypes.c_void_p]
+    lib.lth_batch_free(h)
