@@ -266,24 +266,43 @@ def batched_cpu_baseline(dims, seconds):
 
 def bench_batched(args):
     """512 independent matmul nests of uneven sizes (M, N, K log-uniform in
-    [32, 2048], multiples of 16), bf16 (and fp16) operands, fp32 accumulation:
-    one launch per GEMM (plain) against the autotuned grouped persistent
-    launch replayed from a hipGraph.  A step is one pass over the whole
-    batch."""
-    from loop_tool_amd import batched as B, hip
+    [32, 2048], multiples of 16), bf16 (and fp16) operands, fp32 accumulation.
+    The tuned form: per shape bucket a schedule is tuned in the context of
+    the whole batch and stored in the hip backend's tuned store; the batch
+    takes its tile configurations from the stored schedules and runs as one
+    persistent grouped launch replayed from a hipGraph.  Plain forms: one
+    hip-backend Compiled::run per matmul nest (default schedule, float32
+    operands in HBM staged to 16 bits by each plan), and one launch of the
+    library's default tile per GEMM.  A step is one pass over the batch."""
+    import tempfile
+    import loop_tool_amd as lt_mod
+    from loop_tool_amd import autotune as A, batched as B, hip
     hip.init()
+    L = lt_mod.lib
     dims = B.shapes(n=512, seed=6)
     stream = hip.Stream()
     flops = sum(2 * m * n * k for m, n, k in dims)
     # algorithmic HBM bytes: 16-bit operands read once, fp32 output written once
     nbytes = sum(2 * (m * k + k * n) + 4 * m * n for m, n, k in dims)
+    store = A.TunedStore(L, tempfile.mkdtemp(prefix="lt_bench_tuned_"))
     out = {}
     for dt in ("bf16", "f16"):
         b = B.Batch(dims, dtype=dt, seed=6, stream=stream.h)
         b.run_plain(B.DEFAULT_CFG)
         stream.synchronize()
         plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
-        cfg, report = B.tune_buckets(b, iters=3)
+        t0 = time.perf_counter()
+        pn = B.PlainNests(L, b, dtype=dt)
+        compile_s = time.perf_counter() - t0
+        pn.run()
+        stream.synchronize()
+        plain_hip_ms = B.time_ms(pn.run, iters=5, stream=stream.h)
+        del pn
+        cfg, report = B.tune_buckets(b, iters=3, store=store, L=L)
+        stored = B.configs_from_store(L, store, dims, dt)
+        assert stored == cfg or [(B.TILES[c], B.TILE_K[c]) for c in stored] == [
+            (B.TILES[c], B.TILE_K[c]) for c in cfg]
+        b.set_configs(stored)
         grouped_ms = B.time_ms(lambda: b.run_grouped(), iters=10, stream=stream.h)
         b.capture()
         if dt == "bf16":
@@ -299,14 +318,20 @@ def bench_batched(args):
         # the dominant (only) kernel of a replay: HIP events on its stream
         graph_ms = B.time_ms(b.replay, iters=max(args.steps, 20), stream=stream.h)
         e, e16 = batch_errors(b)
-        out[dt] = dict(plain_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
+        out[dt] = dict(plain_hip_backend_ms=round(plain_hip_ms, 5),
+                       plain_hip_backend_compile_s=round(compile_s, 2),
+                       plain_library_ms=round(plain_ms, 5), grouped_ms=round(grouped_ms, 5),
                        graph_ms=round(graph_ms, 5),
-                       speedup_vs_plain=round(plain_ms / graph_ms, 3),
+                       speedup_vs_plain_hip_backend=round(plain_hip_ms / graph_ms, 3),
+                       speedup_vs_plain_library=round(plain_ms / graph_ms, 3),
                        work_items=b.num_items(), tuning=report,
+                       configs_from="schedules stored per shape bucket (%d) in the hip "
+                                    "backend's tuned store" % len(report["buckets"]),
                        max_err_vs_f64=e, max_err_vs_f64_of_rounded_operands=e16,
                        error_sample="48 random rows (all rows when fewer) of every GEMM, "
                                     "after the graph replays")
         b.free()
+    A.TunedStore.reset(L)
     kern_ms = out["bf16"]["graph_ms"]
     achieved = nbytes / (kern_ms * 1e-3) / 1e9
     line = {

@@ -122,6 +122,8 @@ int ltc_looptree_loop(const ltc_looptree *lt, int ref, int *var, int *var_depth,
                       int64_t *size, int64_t *tail);
 int ltc_looptree_var_name(const ltc_looptree *lt, int var, char *buf, int cap);
 int ltc_looptree_node(const ltc_looptree *lt, int ref);   /* IR node of a leaf */
+/* vars (ids, as in ltc_looptree_loop) of a leaf's IR node, in its order */
+int ltc_looptree_node_vars(const ltc_looptree *lt, int ref, int *out, int cap);
 int ltc_looptree_node_op(const ltc_looptree *lt, int ref, char *buf, int cap);
 int ltc_looptree_annotation(const ltc_looptree *lt, int ref, char *buf, int cap);
 int ltc_looptree_num_inputs(const ltc_looptree *lt);

@@ -46,6 +46,25 @@ int lth_time(const ltc_compiled *cc, void *const *io, int n_io, int warmup, int 
 int lth_time_stage(const ltc_compiled *cc, void *const *io, int n_io, int stage, int warmup,
                    int iters, float *ms);
 
+/* ---- tuned-schedule store.  No counterpart in the reference, whose
+ * schedules live only in memory; the file format is the IR text of
+ * serialize/deserialize (include/loop_tool/serialization.h).  The hip
+ * backend's compile looks every program up here and, when a schedule of the
+ * same program is stored, lowers that schedule instead (the Compiled's
+ * string property "tuned_schedule" then holds the key, and its dump shows
+ * the schedule).  One IR text file per program under <dir>/gfx950/<key>.ir;
+ * <dir> is the last lth_tuned_set_dir, else $LT_HIP_TUNED_DIR, else
+ * $HOME/.cache/loop_tool_amd/tuned.  LT_HIP_TUNED=0 turns lookups off. */
+int lth_tuned_set_dir(const char *dir);
+/* the program's key (hex): its IR text with var names replaced by positions,
+ * hashed */
+int lth_schedule_key(const ltc_looptree *program, char *buf, int cap);
+/* store `schedule` (a LoopTree of the same program) as the program's tuned
+ * schedule; refused when nodes, vars or var sizes differ */
+int lth_tuned_put(const ltc_looptree *program, const ltc_looptree *schedule);
+/* the stored schedule of the program, or NULL when there is none */
+ltc_looptree *lth_tuned_get(const ltc_looptree *program);
+
 /* ---- allocation-free launch.  No counterpart in the reference: for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  As lth_run_on, but the

@@ -5,13 +5,15 @@ and rounded to multiples of 16 (SplitMix64 seed), 16-bit (bf16 or fp16)
 operands resident in HBM, float32 accumulation and output.
 
 Each GEMM is a loop_tool matmul nest; the autotuner picks a schedule per
-shape bucket -- the nest split by the tile (BM, BN, BK) and annotated
-`tile=BMxBN bk=BK` -- by timing the whole batch's grouped launch under
-candidate tile configurations, and stores the tuned LoopTree (IR text) per
-bucket.  The tuned batch then
-runs as one persistent launch over every output tile of every GEMM, captured
-once into a hipGraph; the plain form is one launch per GEMM with the default
-128x128 tile.
+shape bucket -- the bucket's representative matmul nest split by the tile
+(BM, BN, BK) -- by timing the whole batch's grouped launch under candidate
+tile configurations, and stores that schedule in the hip backend's tuned
+store.  A batch takes its tile configurations from the stored schedules:
+each is lowered by the hip backend, and the tile it chooses is the GEMMs'
+configuration.  The tuned batch runs as one persistent launch over every
+output tile of every GEMM, captured once into a hipGraph.  The plain forms
+are one hip-backend Compiled::run per matmul nest (PlainNests) and one
+launch of the library's tile per GEMM.
 """
 import ctypes as C
 import math
@@ -174,8 +176,10 @@ def tune_buckets(batch, grid=0, iters=3, store=None, L=None, rounds=1):
     bucket (most flops first), keep any configuration that makes the whole
     grouped launch faster by more than 1%.  (Timing a bucket alone favours
     small tiles, which only pay off when the bucket has the GPU to itself.)
-    Returns the per-GEMM configurations and a report; stores one tuned
-    LoopTree per bucket."""
+    Returns the per-GEMM configurations and a report.  With a `store` (an
+    autotune.TunedStore) and `L`, each bucket's choice is stored as a
+    schedule of the bucket's representative matmul nest (bucket_program):
+    the nest split by the tile and the K tile."""
     groups = {}
     for i, (m, n, k) in enumerate(batch.dims):
         groups.setdefault(bucket(m, n, k), []).append(i)
@@ -213,20 +217,106 @@ def tune_buckets(batch, grid=0, iters=3, store=None, L=None, rounds=1):
         report["buckets"]["%dx%d" % (2 ** key[0], 2 ** key[1])] = dict(
             gemms=len(groups[key]), tile="%dx%dx%d" % (TILES[c] + (TILE_K[c],)))
         if store is not None and L is not None:
-            m, nn, k = batch.dims[groups[key][0]]
-            store.put("matmul-bucket-%dx%d-%s" % (2 ** key[0], 2 ** key[1],
-                                                  "bf16" if batch.dtype == 0 else "f16"),
-                      tuned_tree(L, m, nn, k, c), best, "tile=%dx%d bk=%d" % (
-                          TILES[c] + (TILE_K[c],)))
+            dt = "bf16" if batch.dtype == 0 else "f16"
+            prog = bucket_program(L, key, dt)
+            store.put(prog, tuned_tree(L, prog, c))
     return cfg, report
 
 
-def tuned_tree(L, M, N, K, cfg):
-    """The matmul nest scheduled for tile configuration `cfg`: M and N split
-    by the tile, K split by 32, annotated with the tile for the hip backend."""
-    out, _ = P.matmul(L, M, N, K)
+def bucket_program(L, key, dtype="bf16"):
+    """The representative matmul nest of a shape bucket: M = 2^key[0] and
+    N = 2^key[1], each at least 512, K = 512 (large enough that every tile
+    of the library is a split of its loops); fp16 batches annotate it
+    `mfma_f16`, so the two operand types keep schedules of their own."""
+    out, _ = P.matmul(L, max(2 ** key[0], 512), max(2 ** key[1], 512), 512)
     t = out.loop_tree
+    return t if dtype == "bf16" else t.annotate(t.roots[0], "mfma_f16")
+
+
+def tuned_tree(L, program, cfg):
+    """A schedule of a matmul nest for tile configuration `cfg`: its M, N
+    and K loops split by the tile's rows, columns and K tile (what the hip
+    lowering reads as the tile of a contraction)."""
+    from .autotune import split_var, leaves
+    t = program
+    red = leaves(t, "add")[0]
+    m, n = t.node_vars(red)
+    k = next(t.loop(r)["var"] for r in t.loops if t.loop(r)["var"] not in (m, n))
     bm, bn = TILES[cfg]
-    if M > bm:
-        t = t.split(t.roots[0], bm)
-    return t.annotate(t.roots[0], "tile=%dx%d bk=%d mfma_bf16" % (bm, bn, TILE_K[cfg]))
+    for var, size in ((m, bm), (n, bn), (k, TILE_K[cfg])):
+        nt = split_var(t, "add", var, size)
+        if nt is None:
+            raise ValueError("matmul nest too small for the tile %dx%dx%d" % (
+                bm, bn, TILE_K[cfg]))
+        t = nt
+    return t
+
+
+def config_of(L, schedule, dtype="bf16"):
+    """The library's tile configuration that the hip backend lowers
+    `schedule` to: it is compiled for 16-bit operands and its tile read back
+    from the Compiled."""
+    backend = "hip_bf16" if dtype == "bf16" else "hip"
+    t = schedule
+    if dtype != "bf16" and "mfma_f16" not in t.dump():
+        t = t.annotate(t.roots[0], "mfma_f16")
+    cc = L.compile(t, backend)
+    got = (cc.int_property("tile_m"), cc.int_property("tile_n"), cc.int_property("bk"))
+    for c, (tile, bk) in enumerate(zip(TILES, TILE_K)):
+        if tile + (bk,) == got:
+            return c
+    raise ValueError("no tile configuration of the batch library is %dx%dx%d" % got)
+
+
+def configs_from_store(L, store, dims, dtype="bf16"):
+    """Per-GEMM tile configurations from the schedules stored for the
+    GEMMs' shape buckets (None where a bucket has no stored schedule)."""
+    by_bucket = {}
+    out = []
+    for m, n, k in dims:
+        key = bucket(m, n, k)
+        if key not in by_bucket:
+            sched = store.get(bucket_program(L, key, dtype))
+            by_bucket[key] = None if sched is None else config_of(L, sched, dtype)
+        out.append(by_bucket[key])
+    return out
+
+
+class PlainNests:
+    """The plain form of the batch: one matmul nest per GEMM compiled by the
+    hip backend (default schedule, 16-bit MFMA operands staged from float32
+    inputs in HBM by the plan itself), launched one after another on one
+    stream with lth_run_on and shared scratch.  Compiles run in threads."""
+
+    def __init__(self, L, batch, dtype="bf16", threads=16):
+        import concurrent.futures as cf
+        self.batch = batch
+        backend = "hip_bf16" if dtype == "bf16" else "hip"
+
+        def compile_one(d):
+            m, n, k = d
+            out, _ = P.matmul(L, m, n, k)
+            t = out.loop_tree
+            if dtype != "bf16":
+                t = t.annotate(t.roots[0], "mfma_f16")
+            return L.compile(t, backend)
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            self.cc = list(ex.map(compile_one, batch.dims))
+        sizes = [hip.scratch_sizes(c) for c in self.cc]
+        nsc = max(len(s) for s in sizes)
+        self.scratch = [DeviceArray(max(s[j] for s in sizes if len(s) > j)) for j in range(nsc)]
+        self.io = []
+        for i, (m, n, k) in enumerate(batch.dims):
+            a, b = batch.host[i]
+            self.io.append([DeviceArray.from_numpy(a), DeviceArray.from_numpy(b),
+                            DeviceArray(m * n)])
+        hip.synchronize()
+
+    def run(self):
+        for cc, io in zip(self.cc, self.io):
+            hip.run_on(cc, io, self.scratch[:len(hip.scratch_sizes(cc))], self.batch.stream)
+
+    def result(self, i):
+        self.batch.synchronize()
+        m, n, _ = self.batch.dims[i]
+        return self.io[i][2].download((m, n))

@@ -98,6 +98,7 @@ class Library:
         s("ltc_looptree_var_name", C.c_int, _P, C.c_int, C.c_char_p, C.c_int)
         s("ltc_looptree_node", C.c_int, _P, C.c_int)
         s("ltc_looptree_node_op", C.c_int, _P, C.c_int, C.c_char_p, C.c_int)
+        s("ltc_looptree_node_vars", C.c_int, _P, C.c_int, C.POINTER(C.c_int), C.c_int)
         s("ltc_looptree_annotation", C.c_int, _P, C.c_int, C.c_char_p, C.c_int)
         s("ltc_looptree_num_inputs", C.c_int, _P)
         s("ltc_looptree_num_outputs", C.c_int, _P)
@@ -556,6 +557,11 @@ class LoopTree(_Handle):
 
     def node(self, ref):
         return self.L.check_int(self.L.lib.ltc_looptree_node(self.h, ref))
+
+    def node_vars(self, ref):
+        """Var ids of a leaf's IR node, in the node's order (the last is
+        the contiguous dim of its value)."""
+        return self._ints(self.L.lib.ltc_looptree_node_vars, ref)
 
     def annotation(self, ref):
         return self.L.string(self.L.lib.ltc_looptree_annotation, self.h, ref)

@@ -405,6 +405,14 @@ int ltc_looptree_node(const ltc_looptree *lt, int ref) {
   return guard_int([&] { return (int)lt->lt.node(ref); });
 }
 
+int ltc_looptree_node_vars(const ltc_looptree *lt, int ref, int *out, int cap) {
+  return guard_int([&] {
+    const auto vars = lt->lt.ir.node(lt->lt.node(ref)).vars();
+    for (int i = 0; i < (int)vars.size() && i < cap; ++i) out[i] = (int)vars[i];
+    return (int)vars.size();
+  });
+}
+
 int ltc_looptree_node_op(const ltc_looptree *lt, int ref, char *buf, int cap) {
   return guard_int([&] {
     return copy_out(loop_tool::dump(lt->lt.ir.node(lt->lt.node(ref)).op()), buf, cap);

@@ -4,11 +4,15 @@
 #include <string>
 
 #include "hip/hip_backend.h"
+#include "hip/tuned_store.h"
 #include "loop_tool_hip.h"
 #include "lt_hip.h"
 
 struct ltc_compiled {
   std::shared_ptr<loop_tool::Compiled> cc;
+};
+struct ltc_looptree {
+  loop_tool::LoopTree lt;
 };
 
 namespace {
@@ -86,6 +90,35 @@ int lth_scratch_sizes(const ltc_compiled *cc, int64_t *out, int cap) {
     for (int i = 0; i < (int)s.size() && i < cap; ++i) out[i] = s[i];
     return (int)s.size();
   });
+}
+
+int lth_tuned_set_dir(const char *dir) {
+  return guarded([&] {
+    loop_tool::hip::set_tuned_dir(dir ? dir : "");
+    return 0;
+  });
+}
+
+int lth_schedule_key(const ltc_looptree *program, char *buf, int cap) {
+  return guarded([&] { return copy_str(loop_tool::hip::schedule_key(program->lt), buf, cap); });
+}
+
+int lth_tuned_put(const ltc_looptree *program, const ltc_looptree *schedule) {
+  return guarded([&] {
+    loop_tool::hip::tuned_put(program->lt, schedule->lt);
+    return 0;
+  });
+}
+
+ltc_looptree *lth_tuned_get(const ltc_looptree *program) {
+  try {
+    g_err.clear();
+    auto t = loop_tool::hip::tuned_get(program->lt, nullptr);
+    return t ? new ltc_looptree{*t} : nullptr;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+  }
+  return nullptr;
 }
 
 int lth_run_on(const ltc_compiled *cc, void *const *io, int n_io, void *const *scratch,

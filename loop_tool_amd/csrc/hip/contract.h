@@ -37,6 +37,7 @@ struct Contraction {
   std::vector<VarInfo> I, J, B, K;
   int64_t MD = 1, ND = 1, KD = 1, BD = 1;
   std::map<int, int64_t> ostride;  // output element stride per var sym id
+  IR::NodeRef red_node = -1;       // the add reduction (its loops: the schedule)
 };
 
 // `lin` (a C expression) decomposed row-major over vars (the last fastest)

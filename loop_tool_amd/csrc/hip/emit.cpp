@@ -91,7 +91,9 @@ Plan build_plan(const Dataflow &df, const Knobs &knobs) {
     if (mat.count(nr) || df.nodes.at(nr).op == Operation::write) targets.push_back(nr);
 
   int k = 0;
+  const int block0 = block;
   for (auto target : targets) {
+    block = block0;
     PointwiseEmitter em(df, mat, input_args);
     Stage st;
     st.kernel = "lt_pw" + std::to_string(k++);
@@ -116,20 +118,34 @@ Plan build_plan(const Dataflow &df, const Knobs &knobs) {
     sig << "float* __restrict__ out";
     st.buffers.push_back(out_buf);
     st.writes.push_back(out_buf);
+    // the schedule's split of the target's contiguous var: its innermost
+    // extent (1, 2 or 4) sets the elements per lane, the loop above it
+    // (64..1024 lanes, a multiple of 64) the workgroup size
+    int vec = knobs.vec, tblock = block;
+    const auto &tvars = df.nodes.at(target).vars;
+    if (!tvars.empty()) {
+      const int csym = df.sym(tvars.back()).id();
+      const int64_t e = df.split_extent(target, csym), t = df.split_extent(target, csym, 1);
+      if (vec == 0 && (e == 1 || e == 2 || e == 4)) vec = (int)e;
+      if (!knobs.block_set && t >= 64 && t <= 1024 && t % 64 == 0) tblock = (int)t;
+    }
     int64_t threads = 0;
-    src << em.kernel(st.kernel, target, sig.str(), "out", block, &threads, knobs.vec, int32)
+    src << em.kernel(st.kernel, target, sig.str(), "out", tblock, &threads, vec, int32)
         << "\n";
+    block = tblock;
     int64_t blocks = (threads + block - 1) / block;
     ASSERT(blocks < (1LL << 31)) << "grid too large";
     st.grid[0] = (unsigned)blocks;
     st.block[0] = block;
+    plan.config["block"] = block;
+    if (vec) plan.config["vec"] = vec;
     // algorithmic bytes: every input element read once, the target written once
     int64_t bytes = df.numel(df.nodes.at(target).vars) * 4;
     for (int i = 0; i < n_in; ++i) bytes += df.numel(df.nodes.at(df.ir().inputs()[i]).vars) * 4;
     st.bytes = bytes;
     plan.stages.push_back(st);
     desc << "  stage " << st.kernel << ": " << df.ir().dump(target) << " over "
-         << threads << " elements\n";
+         << threads << " lanes of " << block << " per workgroup" << (vec ? ", " + std::to_string(vec) + " elements per lane at most" : std::string()) << "\n";
   }
   plan.source = src.str();
   plan.description = desc.str();

@@ -1,6 +1,7 @@
 // HIP source emitters for the nest classes of the hip backend.
 #pragma once
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,9 @@ struct Plan {
   std::vector<Stage> stages;
   std::string source;          // all kernels of the plan, one module
   std::string description;     // human-readable summary (Compiled::dump)
+  // the configuration the lowering chose (tile_m, tile_n, bk, vec, block,
+  // ...), exported as the Compiled's int_properties
+  std::map<std::string, int64_t> config;
 };
 
 // Knobs read from LoopTree annotations (see hip_backend.cpp).
@@ -43,6 +47,7 @@ struct Knobs {
   bool bf16 = false;           // "mfma_bf16" operand precision for contractions
   bool f16 = false;            // "mfma_f16"
   int block = 256;             // "block=N"
+  bool block_set = false;      // block came from an annotation
   int vec = 0;                 // "vec=N" elements per thread (0 = auto)
   int tile_m = 0, tile_n = 0;  // "tile=MxN" for contractions (0 = auto)
   int bk = 0;                  // "bk=N": K tile of 16-bit contractions (32, 64)

@@ -73,7 +73,8 @@ bool affine(const Expr &e0, std::map<int, int64_t> &coef, int64_t &c) {
 
 }  // namespace
 
-bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
+bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
+  Knobs knobs = knobs_in;
   if (df.writes.size() != 1) return false;
   const auto &W = df.nodes.at(df.writes[0]);
   const auto &win = W.inputs.at(0);
@@ -172,6 +173,22 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
       if (std::find(kvars.begin(), kvars.end(), v) == kvars.end()) kvars.push_back(v);
   }
   std::vector<VarInfo> K = infos(kvars, "vk");
+  // tile sides from the schedule: a single row (column) var split into
+  // nested loops above the reduction gives the tile's rows (columns) as its
+  // innermost extent (64 to 256, a multiple of 64); a tile=MxN annotation
+  // takes precedence
+  if (knobs.tile_m == 0) {
+    auto side = [&](const std::vector<VarInfo> &U) -> int {
+      if (U.size() != 1) return 0;
+      const int64_t s = df.split_extent(R.nr, U[0].sym);
+      return s >= 64 && s <= 256 && s % 64 == 0 ? (int)s : 0;
+    };
+    const int sm = side(I), sn = side(J);
+    if (sm || sn) {
+      knobs.tile_m = sm ? sm : 128;
+      knobs.tile_n = sn ? sn : 128;
+    }
+  }
   int64_t MD = 1, ND = 1, KD = 1, BD = 1;
   for (auto &v : I) MD *= v.size;
   for (auto &v : J) ND *= v.size;
@@ -194,6 +211,7 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
     c.ND = ND;
     c.KD = KD;
     c.BD = BD;
+    c.red_node = R.nr;
     int64_t st = 1;
     for (int d = (int)W.vars.size() - 1; d >= 0; --d) {
       c.ostride[df.sym(W.vars[d]).id()] = st;
@@ -492,6 +510,9 @@ bool try_contract(const Dataflow &df, const Knobs &knobs, Plan &plan) {
     bytes += df.numel(df.nodes.at(df.ir().inputs()[i]).vars) * 4;
   st.bytes = bytes;
   plan.stages.push_back(st);
+  plan.config["tile_m"] = BM;
+  plan.config["tile_n"] = BN;
+  plan.config["bk"] = BK;
   plan.source = k.str();
   std::stringstream desc;
   desc << "[hip] contraction on MFMA (" << (lowp ? (bf16 ? "bf16" : "f16") : "f32")

@@ -480,6 +480,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     }
   const int64_t KI = c.K[kin].size;
   ASSERT(ko_n < (1LL << 31) && KI < (1LL << 31)) << "reduction too large";
+  // the K tile from the schedule: k_in split into nested loops above the
+  // reduction, innermost extent 32 or 64 (a bk=N annotation takes precedence)
+  const int64_t s_bk = knobs.bk > 0 ? 0 : df.split_extent(c.red_node, c.K[kin].sym);
 
   // ---- tile: the largest tile (fewest LDS bytes per flop) that still gives
   // every CU a workgroup; K tile 64 where the LDS of two stages allows ----
@@ -509,6 +512,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     }
     if (knobs.bk > 0) BK = knobs.bk;
   }
+  if (s_bk == 32 || s_bk == 64) BK = (int)s_bk;
   if (BK != 32 && BK != 64) return false;
   if (BM % 64 || BN % 64 || BM > 512 || BN > 512 || BM * BN > 256 * 256) return false;
   if (2 * (BM + BN) * (BK + 16 / EB) * EB > 160 * 1024) return false;  // LDS of a CU
@@ -765,6 +769,10 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   g.flops = 2 * c.MD * c.ND * c.KD * c.BD;
   g.bytes = (SP.total + SQ.total) * EB + c.MD * c.ND * c.BD * 4;
   plan.stages.push_back(g);
+  plan.config["tile_m"] = BM;
+  plan.config["tile_n"] = BN;
+  plan.config["bk"] = BK;
+  plan.config["waves"] = pipe ? WGM * WGN : 4;
   plan.source = k.str();
   std::stringstream desc;
   desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : f32 ? "f32" : "f16")

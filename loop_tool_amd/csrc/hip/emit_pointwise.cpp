@@ -332,8 +332,10 @@ struct PointwiseEmitter::Impl {
       auto name = fresh("i");
       names[it.id()] = name;
       extents[it.id()] = size;
-      auto *loop = b->child("for (" + ity + " " + name + " = 0; " + name + " < " +
-                                std::to_string(size) + "; ++" + name + ") {",
+      // a loop the schedule marked "unroll" is unrolled completely
+      auto *loop = b->child(std::string(L.unroll ? "#pragma unroll\n" : "") + "for (" + ity +
+                                " " + name + " = 0; " + name + " < " + std::to_string(size) +
+                                "; ++" + name + ") {",
                             "}");
       loop->bound.insert(it.id());
       Block *saved = b->open_child;

@@ -109,11 +109,20 @@ bool try_rowreduce(const Dataflow &df, const Knobs &knobs, Plan &plan) {
   }
 
   // launch shape: T lanes x VEC elements x NK chunks covers the row
-  int vec = knobs.vec > 0 ? knobs.vec : (C % 4 == 0 ? 4 : 1);
+  // the schedule's split of the column var over the write: its innermost
+  // extent (1, 2 or 4) sets the values per vector, the loop above it (64 to
+  // 1024, a multiple of 64) the lanes per row
+  const int csym = df.sym(cvar).id();
+  const int64_t s_vec = df.split_extent(W.nr, csym), s_lanes = df.split_extent(W.nr, csym, 1);
+  int vec = knobs.vec > 0 ? knobs.vec
+            : (s_vec == 1 || s_vec == 2 || s_vec == 4) ? (int)s_vec
+                                                       : (C % 4 == 0 ? 4 : 1);
   if (vec != 1 && vec != 2 && vec != 4) vec = 1;
   if (C % vec) vec = 1;
   int T = knobs.block;
-  if (T == 256) {
+  if (!knobs.block_set && s_lanes >= 64 && s_lanes <= 1024 && s_lanes % 64 == 0)
+    T = (int)s_lanes;
+  else if (!knobs.block_set) {
     // default: two vectors (8 values) per lane, as many lanes as that needs
     // up to 1024 (measured fastest for rows of 1K-8K floats: more rows in
     // flight per CU hide the two reductions' barriers)
@@ -414,6 +423,8 @@ __device__ __forceinline__ float lt_comb_min(float a, float b) { return a < b ? 
   ASSERT(rows < (1LL << 31)) << "too many rows";
   st.grid[0] = (unsigned)rows;
   st.block[0] = T;
+  plan.config["vec"] = vec;
+  plan.config["block"] = T;
   int64_t bytes = rows * C * 4;  // output
   for (int i = 0; i < df.num_inputs; ++i)
     bytes += df.numel(df.nodes.at(df.ir().inputs()[i]).vars) * 4;

@@ -60,6 +60,7 @@
 #include "lt_hip.h"
 #include "nest.h"
 #include "hip_backend.h"
+#include "tuned_store.h"
 
 namespace loop_tool {
 namespace hip {
@@ -282,6 +283,7 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
         ASSERT(k.bk == 32 || k.bk == 64) << "knob " << tok << ": bk=N must be 32 or 64";
       } else if (key == "block") {
         k.block = number(tok, v);
+        k.block_set = true;
       } else if (key == "vec") {
         k.vec = number(tok, v);
         ASSERT(k.vec == 1 || k.vec == 2 || k.vec == 4) << "knob " << tok << ": vec=N must be 1, 2 or 4";
@@ -316,6 +318,7 @@ HipCompiled::HipCompiled(const LoopTree &lt, bool bf16_variant) {
   bool cached = false;
   code_ = compile_source(plan_.source, "lt_module.hip", &cached);
   int_properties["stages"] = (int)plan_.stages.size();
+  for (const auto &p : plan_.config) int_properties[p.first] = p.second;
   int_properties["code_cached"] = cached ? 1 : 0;
   std::string kinds;
   for (const auto &s : plan_.stages) kinds += (kinds.empty() ? "" : ",") + s.kind;
@@ -486,15 +489,24 @@ std::vector<int64_t> HipCompiled::scratch_sizes() const {
 }
 
 std::string HipCompiled::dump() const {
-  return plan_.description + "\n" + plan_.source;
+  return plan_.description + (schedule_note.empty() ? "" : schedule_note) + "\n" + plan_.source;
 }
 
 // ------------------------------------------------------------ registration ----
 struct HipBackend : public Backend {
   bool bf16;
   HipBackend(std::string name, bool bf16_) : Backend(std::move(name)), bf16(bf16_) {}
+  // A tuned schedule stored for this program (tuned_store.h) is lowered in
+  // place of the tree as given; the Compiled names it in its properties and
+  // shows it in dump().
   std::unique_ptr<Compiled> compile_impl(const LoopTree &lt) const override {
-    return std::make_unique<HipCompiled>(lt, bf16);
+    std::string key;
+    auto tuned = tuned_get(lt, &key);
+    if (!tuned) return std::make_unique<HipCompiled>(lt, bf16);
+    auto cc = std::make_unique<HipCompiled>(*tuned, bf16);
+    cc->string_properties["tuned_schedule"] = key;
+    cc->schedule_note = "[hip] tuned schedule " + key + " from the store:\n" + tuned->dump();
+    return cc;
   }
   // accepts CPU-addressable memory (staged) as well as device pointers
   int hardware_requirement() const override { return 1 << 0; }

@@ -50,6 +50,8 @@ class HipCompiled : public Compiled {
   }
 
   const Plan &plan() const { return plan_; }
+  // the stored schedule this Compiled was lowered from (dump() shows it)
+  std::string schedule_note;
   int64_t algorithmic_bytes() const { return algorithmic_bytes_; }
   int64_t algorithmic_flops() const { return algorithmic_flops_; }
   void *stream() const {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <sstream>
 
 namespace loop_tool {
 namespace hip {
@@ -122,8 +123,13 @@ std::unique_ptr<Dataflow> analyze(const LoopTree &lt) {
     std::vector<RedLoop> chain;
     for (auto p = c.lt.parent(ref); p != -1; p = c.lt.parent(p)) {
       auto l = c.lt.loop(p);
+      auto &sizes = n.loops[c.var_to_sym.at(l.var).id()];
+      sizes.insert(sizes.begin(), l.size);
       if (!red.count(l.var)) continue;
-      chain.push_back(RedLoop{l.var, l.size, l.tail, c.inner_sizes.at(p)});
+      std::stringstream annot(c.lt.annotation(p));
+      bool unroll = false;
+      for (std::string tok; annot >> tok;) unroll |= tok == "unroll";
+      chain.push_back(RedLoop{l.var, l.size, l.tail, c.inner_sizes.at(p), unroll});
     }
     std::reverse(chain.begin(), chain.end());
     for (auto v : n.red_vars) {

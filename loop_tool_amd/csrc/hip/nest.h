@@ -25,6 +25,7 @@ struct RedLoop {
   int64_t size;
   int64_t tail;
   int64_t inner;  // extent of the same var covered by loops below this one
+  bool unroll = false;  // the loop carries the "unroll" annotation
 };
 
 struct DFInput {
@@ -42,6 +43,10 @@ struct DFNode {
   int input_index = -1;               // position in IR::inputs() for reads
   int output_index = -1;              // position in IR::outputs() for writes
   int consumers = 0;                  // number of uses by other DF nodes
+  // the node's loops in the LoopTree: per var symbol id, the sizes of the
+  // loops over that var above the node's leaf, outer to inner (two or more
+  // when the schedule split the var)
+  std::map<int, std::vector<int64_t>> loops;
 };
 
 struct Dataflow {
@@ -58,6 +63,16 @@ struct Dataflow {
     int64_t n = 1;
     for (auto v : vars) n *= size(v);
     return n;
+  }
+  // Where the schedule split var `sym` into nested loops above node nr's
+  // leaf: the size of the innermost of them (level 0), or of the one above
+  // it (level 1, when there are three or more); 0 where it did not.
+  int64_t split_extent(IR::NodeRef nr, int sym, int level = 0) const {
+    auto n = nodes.find(nr);
+    if (n == nodes.end()) return 0;
+    auto l = n->second.loops.find(sym);
+    if (l == n->second.loops.end() || (int)l->second.size() < 2 + level) return 0;
+    return l->second[l->second.size() - 1 - level];
   }
 };
 

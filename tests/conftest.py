@@ -5,11 +5,16 @@ and the benchmark's CPU-baseline leg load them.
 """
 import os
 import sys
+import tempfile
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
+
+# the hip backend looks every program up in its tuned-schedule store: tests
+# start from an empty one (tests that tune use directories of their own)
+os.environ.setdefault("LT_HIP_TUNED_DIR", tempfile.mkdtemp(prefix="lt_tuned_session_"))
 
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liblt_oracle.so")
 REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libloop_tool_ref.so")

@@ -78,24 +78,53 @@ def test_grouped_and_graph_match_plain(bat, hip_dev, cfgs):
 
 
 def test_tuned_batch_vs_f64(bat, hip_dev, lt, tmp_path):
+    """Tune the buckets, store one schedule per bucket, and build a fresh
+    batch from the stored schedules: its configurations are the tuned ones,
+    its graph replay equals one launch per GEMM of those configurations bit
+    for bit, and every GEMM is within the bars of float64."""
     from loop_tool_amd.autotune import TunedStore
     dims = bat.shapes(n=48, seed=5, lo=32, hi=512)
     b = bat.Batch(dims, dtype="bf16", seed=9)
-    store = TunedStore(str(tmp_path / "tuned.jsonl"))
-    cfg, report = bat.tune_buckets(b, iters=2, store=store, L=lt)
-    assert set(cfg) <= set(range(len(bat.TILES)))
-    assert len(store.entries) == len(report["buckets"])
-    assert report["tuned_ms"] <= min(report["uniform_ms"].values()) * 1.0001
-    # stored schedules deserialize into LoopTrees of the matmul nest
-    for e in store.entries.values():
-        t = lt.deserialize(e["ir"])
-        assert "tile=" in t.dump()
-    b.capture()
-    b.replay()
-    hip_dev.synchronize()
+    store = TunedStore(lt, str(tmp_path / "tuned"))
+    try:
+        cfg, report = bat.tune_buckets(b, iters=2, store=store, L=lt)
+        assert set(cfg) <= set(range(len(bat.TILES)))
+        assert len(store.keys()) == len(report["buckets"])
+        assert report["tuned_ms"] <= min(report["uniform_ms"].values()) * 1.0001
+        stored = bat.configs_from_store(lt, store, dims)
+        want = [(bat.TILES[c], bat.TILE_K[c]) for c in cfg]
+        assert [(bat.TILES[c], bat.TILE_K[c]) for c in stored] == want
+        b2 = bat.Batch(dims, dtype="bf16", seed=9)
+        b2.set_configs(stored)
+        b2.run_plain(-1)
+        b2.synchronize()
+        plain = [b2.result(i) for i in range(len(dims))]
+        for c in b2.C:
+            c.fill_zero()
+        b2.capture()
+        b2.replay()
+        b2.synchronize()
+        for i in range(len(dims)):
+            assert np.array_equal(b2.result(i), plain[i]), (i, dims[i])
+            e, e16 = b2.check(i)
+            assert e16 <= 1e-5 and e <= 1e-2, (i, dims[i], e, e16)
+    finally:
+        TunedStore.reset(lt)
+
+
+def test_plain_nests_match_the_batch(bat, hip_dev, lt):
+    """The plain form of the batch -- one hip-backend Compiled per matmul
+    nest, launched one after another -- against float64 of the rounded
+    operands, GEMM by GEMM."""
+    dims = bat.shapes(n=24, seed=5, lo=32, hi=512)
+    b = bat.Batch(dims, dtype="bf16", seed=9)
+    pn = bat.PlainNests(lt, b)
+    pn.run()
     for i in range(len(dims)):
-        e, e16 = b.check(i)
-        assert e16 <= 1e-5 and e <= 1e-2, (i, dims[i], e, e16)
+        a, bb = b.host[i]
+        ref16 = P.round_bf16(a).astype(np.float64) @ P.round_bf16(bb).astype(np.float64)
+        y = pn.result(i)
+        assert np.abs(y - ref16).max() / np.abs(ref16).max() <= 1e-5, (i, dims[i])
 
 
 def test_batch_rejects_bad_sizes(bat, hip_dev):
