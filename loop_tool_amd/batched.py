@@ -224,11 +224,12 @@ def tune_buckets(batch, grid=0, iters=3, store=None, L=None, rounds=1):
 
 
 def bucket_program(L, key, dtype="bf16"):
-    """The representative matmul nest of a shape bucket: M = 2^key[0] and
-    N = 2^key[1], each at least 512, K = 512 (large enough that every tile
-    of the library is a split of its loops); fp16 batches annotate it
+    """The representative matmul nest of a shape bucket (one per bucket, so
+    that each bucket keeps a schedule of its own): M and N are 16 times the
+    bucket's upper bounds 2^key[0] and 2^key[1] (every tile of the library
+    is then a split of their loops), K = 512; fp16 batches annotate it
     `mfma_f16`, so the two operand types keep schedules of their own."""
-    out, _ = P.matmul(L, max(2 ** key[0], 512), max(2 ** key[1], 512), 512)
+    out, _ = P.matmul(L, 16 * 2 ** key[0], 16 * 2 ** key[1], 512)
     t = out.loop_tree
     return t if dtype == "bf16" else t.annotate(t.roots[0], "mfma_f16")
 
