@@ -286,19 +286,27 @@ def bench_batched(args):
     nbytes = sum(2 * (m * k + k * n) + 4 * m * n for m, n, k in dims)
     store = A.TunedStore(L, tempfile.mkdtemp(prefix="lt_bench_tuned_"))
     out = {}
+
+    def progress(msg):
+        print("bench.py batched: " + msg, file=sys.stderr, flush=True)
     for dt in ("bf16", "f16"):
+        progress("%s: operands" % dt)
         b = B.Batch(dims, dtype=dt, seed=6, stream=stream.h)
         b.run_plain(B.DEFAULT_CFG)
         stream.synchronize()
         plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
+        progress("%s: compiling 512 plain nests" % dt)
         t0 = time.perf_counter()
         pn = B.PlainNests(L, b, dtype=dt)
         compile_s = time.perf_counter() - t0
+        progress("%s: compiled in %.1f s; timing" % (dt, compile_s))
         pn.run()
         stream.synchronize()
         plain_hip_ms = B.time_ms(pn.run, iters=5, stream=stream.h)
         del pn
+        progress("%s: tuning the buckets" % dt)
         cfg, report = B.tune_buckets(b, iters=3, store=store, L=L)
+        progress("%s: tuned; timing the graph replay" % dt)
         stored = B.configs_from_store(L, store, dims, dt)
         assert stored == cfg or [(B.TILES[c], B.TILE_K[c]) for c in stored] == [
             (B.TILES[c], B.TILE_K[c]) for c in cfg]
@@ -588,6 +596,9 @@ def main():
     }
     if args.nest == "softmax" and not args.no_conv:
         line["config"]["conv2d_bf16"] = conv_bf16_figures(L, hip)
+    if args.nest == "matmul":
+        line["config"]["launch_floor"] = launch_floor(L, hip)
+        line["config"]["workgroups"] = st_grid(cc)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.nest, inputs_np, shapes, args.cpu_seconds,
                                             bytes_step if bound == "hbm" else 0,
@@ -599,6 +610,25 @@ def main():
     if bad:
         print("bench.py: result outside its error bar: %s" % (bad,), file=sys.stderr)
         sys.exit(3)
+
+
+def st_grid(cc):
+    """Workgroups of each stage of a Compiled (from its plan text)."""
+    import re
+    return [int(x) for x in re.findall(r"grid (\d+)", cc.string_property("grid") or "")] or None
+
+
+def launch_floor(L, hip, iters=200):
+    """The smallest hip-backend kernel: a one-element pointwise nest, launched
+    back to back on its plan's stream; mean milliseconds per launch between
+    HIP events (the floor under any one-kernel plan's step time)."""
+    from loop_tool_amd import programs as P
+    out, _ = P.add_broadcast(L, 1, 1)
+    cc = L.compile(out.loop_tree, "hip")
+    io = [hip.DeviceArray(1), hip.DeviceArray(1), hip.DeviceArray(1)]
+    ms = sorted(hip.time_ms(cc, io, warmup=20, iters=iters) for _ in range(3))[1]
+    return dict(kernel="one-element pointwise nest", ms_per_launch=round(ms, 5),
+                note="HIP events around %d back-to-back launches, median of 3" % iters)
 
 
 def conv_bf16_figures(L, hip):

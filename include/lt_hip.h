@@ -118,6 +118,8 @@ int lt_hip_event_synchronize(void *event);
 /* 1 if every work recorded before the event has completed, 0 if not yet */
 int lt_hip_event_query(void *event);
 int lt_hip_event_elapsed_ms(void *start, void *stop, float *ms);
+/* work enqueued on `stream` after this call waits for `event` */
+int lt_hip_stream_wait_event(void *stream, void *event);
 int lt_hip_graph_begin_capture(void *stream);
 int lt_hip_graph_end_capture(void *stream, void **graph_exec);
 int lt_hip_graph_launch(void *graph_exec, void *stream);

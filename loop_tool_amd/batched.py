@@ -283,26 +283,50 @@ def configs_from_store(L, store, dims, dtype="bf16"):
     return out
 
 
+def _matmul_nest(L, dims, dtype):
+    out, _ = P.matmul(L, *dims)
+    t = out.loop_tree
+    return t if dtype == "bf16" else t.annotate(t.roots[0], "mfma_f16")
+
+
+def _compile_nests(args):
+    """Worker: compile matmul nests with the hip backend, which leaves their
+    code objects in the on-disk code cache.  No GPU is touched."""
+    dims, dtype = args
+    import loop_tool_amd as lt_mod
+    L = lt_mod.lib
+    backend = "hip_bf16" if dtype == "bf16" else "hip"
+    for d in dims:
+        L.compile(_matmul_nest(L, d, dtype), backend)
+    return len(dims)
+
+
+def prewarm_code_cache(dims, dtype="bf16", workers=16):
+    """Compile the matmul nests of `dims` in worker processes into the hip
+    backend's on-disk code cache (hipRTC compiles of one process run one at
+    a time; processes run in parallel), so that compiling them here only
+    loads cached code objects."""
+    import multiprocessing as mp
+    dims = sorted(set(tuple(d) for d in dims))
+    workers = max(1, min(workers, len(dims)))
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return sum(pool.map(_compile_nests, [(dims[i::workers], dtype)
+                                             for i in range(workers)]))
+
+
 class PlainNests:
     """The plain form of the batch: one matmul nest per GEMM compiled by the
     hip backend (default schedule, 16-bit MFMA operands staged from float32
     inputs in HBM by the plan itself), launched one after another on one
-    stream with lth_run_on and shared scratch.  Compiles run in threads."""
+    stream with lth_run_on and shared scratch.  `workers` processes compile
+    the nests into the on-disk code cache first (none: compile here)."""
 
-    def __init__(self, L, batch, dtype="bf16", threads=16):
-        import concurrent.futures as cf
+    def __init__(self, L, batch, dtype="bf16", workers=16):
         self.batch = batch
         backend = "hip_bf16" if dtype == "bf16" else "hip"
-
-        def compile_one(d):
-            m, n, k = d
-            out, _ = P.matmul(L, m, n, k)
-            t = out.loop_tree
-            if dtype != "bf16":
-                t = t.annotate(t.roots[0], "mfma_f16")
-            return L.compile(t, backend)
-        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-            self.cc = list(ex.map(compile_one, batch.dims))
+        if workers:
+            prewarm_code_cache(batch.dims, dtype, workers)
+        self.cc = [L.compile(_matmul_nest(L, d, dtype), backend) for d in batch.dims]
         sizes = [hip.scratch_sizes(c) for c in self.cc]
         nsc = max(len(s) for s in sizes)
         self.scratch = [DeviceArray(max(s[j] for s in sizes if len(s) > j)) for j in range(nsc)]

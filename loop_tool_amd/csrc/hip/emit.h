@@ -67,6 +67,10 @@ struct Knobs {
   int streamk = -1;            // "streamk=0|1": float contractions share K tiles
                                // evenly over the workgroups (-1 = auto: when
                                // the tiles fill the workgroup slots unevenly)
+  int overlap = -1;            // "overlap=0|1": run-ahead operand staging
+  int ksplit = -1;             // "ksplit=0|1": gathered contractions on 32x32
+                               // tiles with K tiles split over the waves
+                               // (-1 = auto: nests under 64 64x64 tiles)
   int igemm_halo = -1;         // "igemm=halo" (1) / "igemm=halo1" (2, one halo
                                // buffer) / "igemm=pipe" (0): staged 16-bit
                                // contractions load the Q rows of all taps of a

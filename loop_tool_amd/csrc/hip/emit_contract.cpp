@@ -231,9 +231,22 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
     BN = 64;
   }
   if (BM % 64 || BN % 64 || BM > 256 || BN > 256) return false;
-  const int BK = lowp ? 32 : 16;
   const int KI = lowp ? 16 : 2;  // k per MFMA
-  const int WAVES_M = 2, WAVES_N = 2, NT = 256;
+  // Nests too small to fill the GPU with 64 x 64 tiles (fewer than 64
+  // workgroups): 32 x 32 tiles whose K tiles are split over the
+  // workgroup's four waves -- each wave multiplies its quarter of every K
+  // tile, and the four partial sums are added through LDS in wave order --
+  // for four times the workgroups and sixteen times the waves, every wave
+  // with a quarter of the MFMAs.  "ksplit=0|1" forces the choice.
+  const int BKW = lowp ? 32 : 16;  // K per wave and K tile
+  const int64_t wg64 = ((MD + 63) / 64) * ((ND + 63) / 64) * BD;
+  const bool ksplit = knobs.ksplit == 1 ||
+                      (knobs.ksplit != 0 && knobs.tile_m == 0 && BM == 64 && BN == 64 &&
+                       wg64 < 64 && KD >= 4 * BKW);
+  const int KS = ksplit ? 4 : 1;
+  if (ksplit) BM = BN = 32;
+  const int BK = KS * BKW;  // K extent of a K tile (the LDS tile)
+  const int WAVES_M = ksplit ? 1 : 2, WAVES_N = ksplit ? 1 : 2, NT = 256;
   const int WM = BM / WAVES_M, WN = BN / WAVES_N;  // per-wave tile
   const int TM = WM / 32, TN = WN / 32;            // 32x32 MFMA tiles per wave
   const int PAD = lowp ? 8 : 1;
@@ -278,7 +291,12 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   k << "  __shared__ " << elt << " sP[2][" << BM << "][" << BK + PAD << "];\n";
   k << "  __shared__ " << elt << " sQ[2][" << BN << "][" << BK + PAD << "];\n";
   k << "  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;\n";
-  k << "  const int wm = wave / " << WAVES_N << ", wn = wave % " << WAVES_N << ";\n";
+  if (ksplit)  // every wave covers the whole tile, over its quarter of K
+    k << "  const int wm = 0, wn = 0, kw = wave * " << BKW << ";\n"
+      << "  __shared__ float lt_red[" << KS - 1 << "][16][64];\n";
+  else
+    k << "  const int wm = wave / " << WAVES_N << ", wn = wave % " << WAVES_N
+      << ", kw = 0;\n";
   k << "  const long i0 = (long)blockIdx.y * " << BM << ", j0 = (long)blockIdx.x * " << BN
     << ";\n";
   // batch decomposition
@@ -433,7 +451,7 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   k << "    const bool more = kt + 1 < " << NKT << ";\n";
   k << "    if (more) {\n      k0 = (kt + 1) * " << BK << ";\n" << loadP << loadQ << "    }\n";
   // MFMA over the current buffer
-  k << "    #pragma unroll\n    for (int kk = 0; kk < " << BK / KI << "; ++kk) {\n";
+  k << "    #pragma unroll\n    for (int kk = 0; kk < " << BKW / KI << "; ++kk) {\n";
   if (lowp) {
     const char *vt = bf16 ? "lt_bf16x8" : "lt_f16x8";
     const char *fn = bf16 ? "__builtin_amdgcn_mfma_f32_32x32x16_bf16"
@@ -441,19 +459,19 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
     k << "      " << vt << " av[" << TM << "], bv[" << TN << "];\n";
     k << "      #pragma unroll\n      for (int a = 0; a < " << TM << "; ++a)\n"
       << "        av[a] = *(const " << vt << "*)&sP[buf][wm * " << WM
-      << " + a * 32 + (lane & 31)][kk * 16 + (lane >> 5) * 8];\n";
+      << " + a * 32 + (lane & 31)][kw + kk * 16 + (lane >> 5) * 8];\n";
     k << "      #pragma unroll\n      for (int b = 0; b < " << TN << "; ++b)\n"
       << "        bv[b] = *(const " << vt << "*)&sQ[buf][wn * " << WN
-      << " + b * 32 + (lane & 31)][kk * 16 + (lane >> 5) * 8];\n";
+      << " + b * 32 + (lane & 31)][kw + kk * 16 + (lane >> 5) * 8];\n";
     k << "      #pragma unroll\n      for (int a = 0; a < " << TM << "; ++a)\n"
       << "      #pragma unroll\n      for (int b = 0; b < " << TN << "; ++b)\n"
       << "        acc[a][b] = " << fn << "(av[a], bv[b], acc[a][b], 0, 0, 0);\n";
   } else {
     k << "      float av[" << TM << "], bv[" << TN << "];\n";
     k << "      #pragma unroll\n      for (int a = 0; a < " << TM << "; ++a)\n"
-      << "        av[a] = sP[buf][wm * " << WM << " + a * 32 + (lane & 31)][kk * 2 + (lane >> 5)];\n";
+      << "        av[a] = sP[buf][wm * " << WM << " + a * 32 + (lane & 31)][kw + kk * 2 + (lane >> 5)];\n";
     k << "      #pragma unroll\n      for (int b = 0; b < " << TN << "; ++b)\n"
-      << "        bv[b] = sQ[buf][wn * " << WN << " + b * 32 + (lane & 31)][kk * 2 + (lane >> 5)];\n";
+      << "        bv[b] = sQ[buf][wn * " << WN << " + b * 32 + (lane & 31)][kw + kk * 2 + (lane >> 5)];\n";
     k << "      #pragma unroll\n      for (int a = 0; a < " << TM << "; ++a)\n"
       << "      #pragma unroll\n      for (int b = 0; b < " << TN << "; ++b)\n"
       << "        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[a][b], 0, 0, 0);\n";
@@ -461,6 +479,15 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   k << "    }\n";
   k << "    if (more) {\n      buf ^= 1;\n" << storeP << storeQ << "    }\n";
   k << "    __syncthreads();\n  }\n";
+  if (ksplit) {
+    // the partial sums of waves 1..3 added to wave 0's in wave order
+    k << "  if (wave > 0) {\n"
+      << "    #pragma unroll\n    for (int r = 0; r < 16; ++r) lt_red[wave - 1][r][lane] = "
+         "acc[0][0][r];\n  }\n"
+      << "  __syncthreads();\n  if (wave > 0) return;\n"
+      << "  #pragma unroll\n  for (int w = 0; w < " << KS - 1 << "; ++w)\n"
+      << "  #pragma unroll\n    for (int r = 0; r < 16; ++r) acc[0][0][r] += lt_red[w][r][lane];\n";
+  }
 
   // ---- epilogue: D layout of 32x32 MFMA: col = lane%32,
   // row = (r/4)*8 + (lane/32)*4 + r%4 for accumulator register r
@@ -513,11 +540,13 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   plan.config["tile_m"] = BM;
   plan.config["tile_n"] = BN;
   plan.config["bk"] = BK;
+  plan.config["ksplit"] = KS;
   plan.source = k.str();
   std::stringstream desc;
   desc << "[hip] contraction on MFMA (" << (lowp ? (bf16 ? "bf16" : "f16") : "f32")
        << " operands, f32 accumulate): D[" << BD << "][" << MD << "][" << ND
-       << "] = P x Q over K=" << KD << ", tile " << BM << "x" << BN << "x" << BK << "\n";
+       << "] = P x Q over K=" << KD << ", tile " << BM << "x" << BN << "x" << BK
+       << (ksplit ? ", K tiles split over 4 waves" : "") << "\n";
   plan.description = desc.str();
   return true;
 }

@@ -23,6 +23,13 @@
 //   bk=N           K tile (32 or 64) of 16-bit contractions
 //   contract=gather  contractions gather their operands in the tile loader
 //                  instead of staging them
+//   overlap=0|1    consecutive runs of a plan whose first stage stages
+//                  operands into scratch overlap that stage with the
+//                  previous run's main kernels (two scratch sets, a second
+//                  stream; default on where the plan allows it)
+//   ksplit=0|1     gathered contractions: 32x32 tiles with each K tile split
+//                  over the workgroup's four waves (default: on for nests
+//                  under 64 tiles of 64x64)
 //   streamk=0|1    staged float contractions: stream-K work split over 512
 //                  workgroups off / on (default: on when the tiles fill the
 //                  workgroup slots unevenly)
@@ -49,6 +56,7 @@
 #include <fstream>
 #include <list>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <unordered_map>
 
@@ -261,6 +269,12 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
       } else if (key == "streamk") {
         if (v == "0" || v == "1") k.streamk = v == "1";
         else bad();
+      } else if (key == "overlap") {
+        if (v == "0" || v == "1") k.overlap = v == "1";
+        else bad();
+      } else if (key == "ksplit") {
+        if (v == "0" || v == "1") k.ksplit = v == "1";
+        else bad();
       } else if (key == "igemm") {
         if (v == "plain") k.igemm_plain = true;
         else if (v == "halo") k.igemm_halo = 1;
@@ -310,6 +324,26 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
   return k;
 }
 
+namespace {
+// The first stage can run ahead of the rest of the plan on a second stream
+// when it reads only inputs and writes only scratch that the later stages
+// read and never write (the operand staging of contractions; not a staging
+// launch that also zeroes output blocks for stream-K).
+bool overlappable(const Plan &plan) {
+  if (plan.stages.size() < 2) return false;
+  const auto &s0 = plan.stages[0];
+  std::set<int> w0(s0.writes.begin(), s0.writes.end());
+  for (int b : s0.writes)
+    if (plan.buffers.at(b).kind != BufferSpec::SCRATCH) return false;
+  for (int b : s0.buffers)
+    if (!w0.count(b) && plan.buffers.at(b).kind != BufferSpec::INPUT) return false;
+  for (size_t k = 1; k < plan.stages.size(); ++k)
+    for (int b : plan.stages[k].writes)
+      if (w0.count(b) || plan.buffers.at(b).kind == BufferSpec::INPUT) return false;
+  return true;
+}
+}  // namespace
+
 // ------------------------------------------------------------ HipCompiled ----
 HipCompiled::HipCompiled(const LoopTree &lt, bool bf16_variant) {
   auto df = analyze(lt);
@@ -324,6 +358,14 @@ HipCompiled::HipCompiled(const LoopTree &lt, bool bf16_variant) {
   for (const auto &s : plan_.stages) kinds += (kinds.empty() ? "" : ",") + s.kind;
   string_properties["kinds"] = kinds;
   string_properties["code"] = plan_.source;
+  std::string grids;  // "<kernel> grid <workgroups> block <threads>" per stage
+  for (const auto &s : plan_.stages)
+    grids += (grids.empty() ? "" : "; ") + s.kernel + " grid " +
+             std::to_string((uint64_t)s.grid[0] * s.grid[1] * s.grid[2]) + " block " +
+             std::to_string(s.block[0] * s.block[1] * s.block[2]);
+  string_properties["grid"] = grids;
+  overlap_ = knobs_.overlap != 0 && overlappable(plan_);
+  int_properties["overlap"] = overlap_ ? 1 : 0;
   int64_t bytes = 0, flops = 0;
   for (const auto &s : plan_.stages) {
     bytes += s.bytes;
@@ -338,7 +380,11 @@ bool runtime_exiting() { return g_exiting.load(); }
 HipCompiled::~HipCompiled() {
   if (!loaded_ || runtime_exiting()) return;
   for (auto p : scratch_) lt_hip_free(p);
+  for (auto p : scratch2_) lt_hip_free(p);
   for (auto &p : staging_) lt_hip_free(p.second.first);
+  for (auto *e : {ev_staged_[0], ev_staged_[1], ev_done_[0], ev_done_[1]})
+    if (e) lt_hip_event_destroy(e);
+  if (side_) lt_hip_stream_destroy(side_);
   if (stream_) lt_hip_stream_destroy(stream_);
   if (module_) lt_hip_module_unload(module_);
 }
@@ -367,15 +413,18 @@ void HipCompiled::load() const {
 // The Compiled's own scratch, allocated on first use by run()/run_stage():
 // run_on() callers (the serving runtime) bring theirs, so a program compiled
 // only for them holds no device memory beyond its code.
-void HipCompiled::ensure_scratch() const {
-  if (!scratch_.empty()) return;
+void HipCompiled::ensure_scratch(int set) const {
+  auto &sc = set ? scratch2_ : scratch_;
+  if (!sc.empty()) return;
   for (const auto &b : plan_.buffers) {
     if (b.kind != BufferSpec::SCRATCH) continue;
     void *p = nullptr;
     hip_check(lt_hip_malloc(&p, b.numel * sizeof(float)), "scratch allocation");
-    scratch_.push_back(p);
+    sc.push_back(p);
   }
 }
+
+
 
 void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream, int first,
                          int last) const {
@@ -392,13 +441,14 @@ void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream
   }
 }
 
-std::vector<void *> HipCompiled::device_buffers(const std::vector<void *> &io) const {
-  ensure_scratch();
+std::vector<void *> HipCompiled::device_buffers(const std::vector<void *> &io, int set) const {
+  ensure_scratch(set);
+  const auto &sc = set ? scratch2_ : scratch_;
   std::vector<void *> bufs;
   size_t scratch_i = 0;
   for (const auto &b : plan_.buffers) {
     if (b.kind == BufferSpec::SCRATCH) {
-      bufs.push_back(scratch_[scratch_i++]);
+      bufs.push_back(sc[scratch_i++]);
     } else {
       size_t pos = b.kind == BufferSpec::INPUT ? b.index : num_inputs() + b.index;
       bufs.push_back(io.at(pos));
@@ -445,7 +495,32 @@ void HipCompiled::run(const std::vector<void *> &memory, bool sync) const {
     io[i] = st.first;
     staged = true;
   }
-  launch(device_buffers(io), stream_);
+  if (overlap_ && !staged) {
+    // Runs alternate between two scratch sets.  The first stage (operand
+    // staging) runs on the side stream as soon as the run that last used
+    // its set has finished; the later stages run on the Compiled's stream
+    // after it.  Run i+1's staging thus overlaps run i's main kernels.
+    if (!side_) {
+      hip_check(lt_hip_stream_create(&side_), "stream");
+      for (int b = 0; b < 2; ++b) {
+        hip_check(lt_hip_event_create(&ev_staged_[b]), "event");
+        hip_check(lt_hip_event_create(&ev_done_[b]), "event");
+        hip_check(lt_hip_event_record(ev_done_[b], stream_), "event record");
+      }
+    }
+    const int set = (int)(runs_++ & 1);
+    auto bufs = device_buffers(io, set);
+    hip_check(lt_hip_stream_wait_event(side_, ev_done_[set]), "stream wait");
+    launch(bufs, side_, 0, 0);
+    hip_check(lt_hip_event_record(ev_staged_[set], side_), "event record");
+    hip_check(lt_hip_stream_wait_event(stream_, ev_staged_[set]), "stream wait");
+    launch(bufs, stream_, 1);
+    hip_check(lt_hip_event_record(ev_done_[set], stream_), "event record");
+  } else {
+    launch(device_buffers(io), stream_);
+    // a later run-ahead staging into set 0 must wait for this run
+    if (side_) hip_check(lt_hip_event_record(ev_done_[0], stream_), "event record");
+  }
   for (const auto &p : copy_out)
     hip_check(lt_hip_memcpy_async(p.second, io[p.first],
                                   plan_.buffers[p.first].numel * sizeof(float), 1, stream_),
@@ -479,6 +554,7 @@ void HipCompiled::run_stage(const std::vector<void *> &device_io, int k, void *s
   ASSERT(k >= 0 && k < (int)plan_.stages.size()) << "no stage " << k;
   ASSERT(device_io.size() >= num_io()) << "expected " << num_io() << " device buffers";
   launch(device_buffers(device_io), stream, k, k);
+  if (side_) hip_check(lt_hip_event_record(ev_done_[0], stream), "event record");
 }
 
 std::vector<int64_t> HipCompiled::scratch_sizes() const {

@@ -63,10 +63,10 @@ class HipCompiled : public Compiled {
 
  private:
   void load() const;
-  void ensure_scratch() const;
+  void ensure_scratch(int set = 0) const;
   void launch(const std::vector<void *> &device_buffers, void *stream, int first = 0,
               int last = -1) const;
-  std::vector<void *> device_buffers(const std::vector<void *> &io) const;
+  std::vector<void *> device_buffers(const std::vector<void *> &io, int set = 0) const;
 
   Plan plan_;
   Knobs knobs_;
@@ -79,6 +79,13 @@ class HipCompiled : public Compiled {
   mutable lt_hip_module module_ = nullptr;
   mutable std::vector<void *> functions_;
   mutable std::vector<void *> scratch_;
+  // overlap of the staging stage with the previous run (see run())
+  bool overlap_ = false;
+  mutable std::vector<void *> scratch2_;
+  mutable void *side_ = nullptr;
+  mutable void *ev_staged_[2] = {nullptr, nullptr};
+  mutable void *ev_done_[2] = {nullptr, nullptr};
+  mutable uint64_t runs_ = 0;
   mutable void *stream_ = nullptr;
   mutable std::unordered_map<size_t, std::pair<void *, size_t>> staging_;
 };

@@ -245,6 +245,10 @@ int lt_hip_event_destroy(void *event) {
 int lt_hip_event_record(void *event, void *stream) {
   return check(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
 }
+int lt_hip_stream_wait_event(void *stream, void *event) {
+  return check(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0),
+               "hipStreamWaitEvent");
+}
 int lt_hip_event_synchronize(void *event) {
   return check(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
 }

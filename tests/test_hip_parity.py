@@ -246,6 +246,32 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     assert normwise(ys[0], f64_bf) <= 1e-5
 
 
+@pytest.mark.parametrize("name,kw", CONTRACT, ids=["%s%d" % (c[0], i) for i, c in enumerate(CONTRACT)])
+@pytest.mark.parametrize("backend", ["hip", "hip_bf16"])
+def test_ksplit_tile_vs_whole_tile(lt, hip_dev, name, kw, backend):
+    """The gathered tile with each K tile split over four waves (32 x 32
+    tiles, partial sums added in wave order) against the same nest on 64 x
+    64 tiles: within float32 rounding of each other (<= 1e-5 normwise), the
+    float path within 1e-5 of the C oracle, 16-bit operands within 1e-5 of
+    float64 of the rounded operands; two runs give the same bits."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    extra = " contract=gather" if backend == "hip_bf16" else ""
+    y1, cc1 = run_backend(lt, out, shapes, inputs, backend, annotate="ksplit=1" + extra)
+    assert "split over 4 waves" in cc1.dump().splitlines()[0]
+    y0, cc0 = run_backend(lt, out, shapes, inputs, backend, annotate="ksplit=0" + extra)
+    assert "split over" not in cc0.dump().splitlines()[0]
+    assert normwise(y1, y0) <= 1e-5
+    if backend == "hip":
+        y_or, _ = run_backend(lt, out, shapes, inputs, "cpp")
+        assert normwise(y1, y_or) <= 1e-5
+    else:
+        f64 = P.reference_output(name, [_bf16_round(a) for a in inputs], **ref_kw(name, kw)).ravel()
+        assert normwise(y1, f64) <= 1e-5
+    y2, _ = run_backend(lt, out, shapes, inputs, backend, annotate="ksplit=1" + extra)
+    assert np.array_equal(y1.view(np.uint32), y2.view(np.uint32))
+
+
 def test_pointwise_past_2_to_the_30_elements(lt, hip_dev):
     """Buffers of more than 2^30 elements take the 64-bit index form of the
     pointwise lowering (32-bit offsets are used only below that size): a
@@ -339,7 +365,8 @@ KNOB_OPTIONS = [["", "tile=64x64", "tile=128x128", "tile=64x128", "tile=128x256"
                 ["", "waves=2", "waves=8"], ["", "lds=swz"], ["", "stage=flat"],
                 ["", "kb=inner", "kb=outer"],
                 ["", "igemm=plain", "igemm=pipe", "igemm=halo", "igemm=halo1"],
-                ["", "contract=staged", "contract=gather"], ["", "streamk=1", "streamk=0"]]
+                ["", "contract=staged", "contract=gather"], ["", "streamk=1", "streamk=0"],
+                ["", "ksplit=1", "ksplit=0"]]
 
 
 def random_knob_cases(n=48, seed=21):
@@ -418,3 +445,40 @@ def test_run_without_sync_on_device_buffers(lt, hip_dev):
     yh = np.zeros_like(y_or)
     cc.run([x, yh], sync=False)
     assert np.array_equal(yh.view(np.uint32), y.view(np.uint32))
+
+
+def test_overlapped_runs_keep_each_run_apart(lt, hip_dev):
+    """A staged bf16 plan overlaps a run's operand staging with the previous
+    run's MFMA kernel (two scratch sets, a second stream).  Runs queued back
+    to back without synchronisation, each on its own inputs and output, and
+    mixed with host-pointer runs and with the plan forced serial
+    (overlap=0), all give the bits of the serial plan."""
+    out, shapes = P.conv2d(lt, 4, 64, 64, 28, 28)
+    fan = 64 * 9
+    sets = [[P.normal(40 + 2 * i, shapes[0], fan ** -0.5),
+             P.normal(41 + 2 * i, shapes[1], fan ** -0.5)] for i in range(5)]
+    t = out.loop_tree
+    cc = lt.compile(t, "hip_bf16")
+    assert cc.int_property("overlap") == 1
+    serial = lt.compile(t.annotate(t.roots[0], "overlap=0"), "hip_bf16")
+    assert serial.int_property("overlap") == 0
+    n_out = t.io_numel()[2]
+    want = []
+    for x, w in sets:
+        y = np.zeros(n_out, np.float32)
+        serial.run([x, w, y])
+        want.append(y)
+    d_in = [[hip_dev.DeviceArray.from_numpy(a) for a in s] for s in sets]
+    d_out = [hip_dev.DeviceArray(n_out) for _ in sets]
+    for rep in range(3):
+        for i in range(len(sets)):
+            cc.run([d_in[i][0].ptr, d_in[i][1].ptr, d_out[i].ptr], sync=False)
+        if rep == 1:  # a host-pointer run in the middle of the queue
+            y = np.zeros(n_out, np.float32)
+            cc.run(sets[2] + [y])
+            assert np.array_equal(y.view(np.uint32), want[2].view(np.uint32))
+        hip_dev.synchronize()
+        for i in range(len(sets)):
+            got = d_out[i].download()
+            assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (rep, i)
+            d_out[i].fill_zero()
