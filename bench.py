@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--slots", type=int, default=8, help="serving: requests in flight")
     ap.add_argument("--annotate", default="",
                     help="hip backend knobs for the nest, e.g. 'tile=128x256'")
+    ap.add_argument("--quick", action="store_true",
+                    help="batched: no plain forms and no tuning (every GEMM on the "
+                         "128x128x32 tile), for profiling the grouped kernel")
     ap.add_argument("--no-conv", action="store_true",
                     help="softmax run: skip the conv2d bf16 kernel and plan figures")
     return ap.parse_args()
@@ -286,12 +289,30 @@ def bench_batched(args):
     nbytes = sum(2 * (m * k + k * n) + 4 * m * n for m, n, k in dims)
     store = A.TunedStore(L, tempfile.mkdtemp(prefix="lt_bench_tuned_"))
     out = {}
+    dtypes = ("bf16",) if args.quick else ("bf16", "f16")
 
     def progress(msg):
         print("bench.py batched: " + msg, file=sys.stderr, flush=True)
-    for dt in ("bf16", "f16"):
+    for dt in dtypes:
         progress("%s: operands" % dt)
         b = B.Batch(dims, dtype=dt, seed=6, stream=stream.h)
+        if args.quick:
+            b.set_configs([6] * len(dims))
+            b.capture()
+            for _ in range(args.warmup):
+                b.replay()
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                b.replay()
+            stream.synchronize()
+            el = time.perf_counter() - t0
+            e, e16 = batch_errors(b)
+            out[dt] = dict(graph_ms=round(el / args.steps * 1e3, 5), configs="128x128x32 tile",
+                           max_err_vs_f64=e, max_err_vs_f64_of_rounded_operands=e16)
+            settle_steps = 0
+            b.free()
+            continue
         b.run_plain(B.DEFAULT_CFG)
         stream.synchronize()
         plain_ms = B.time_ms(lambda: b.run_plain(B.DEFAULT_CFG), iters=10, stream=stream.h)
@@ -359,7 +380,7 @@ def bench_batched(args):
                                 "[32, 2048] (multiples of 16), bf16 operands, fp32 accumulate",
                        gemms=len(dims), algorithmic_flops=flops, algorithmic_bytes=nbytes,
                        settle_steps=settle_steps, settle_s=SETTLE_S,
-                       bf16=out["bf16"], f16=out["f16"],
+                       bf16=out["bf16"], f16=out.get("f16"), quick=bool(args.quick),
                        note="the batch's arithmetic intensity (flops/bytes) is below the "
                             "MI355X ridge point (2.5 PFLOP/s / 8 TB/s), so HBM bounds it"),
         "roofline": dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS,

@@ -44,6 +44,7 @@ struct Item {
 
 // tile configurations of csrc/kernels/gemm.hip (BM x BN x BK)
 constexpr int kNumCfg = 7;
+constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup w runs on XCD w % 8)
 const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128, 128};
 const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256, 128};
 const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32, 32};
@@ -97,7 +98,7 @@ struct lth_batch {
   std::vector<Desc> descs;
   std::vector<int> cfg;
   void *d_descs = nullptr, *d_items = nullptr;
-  void *d_queue = nullptr;  // grouped kernel's item queue: next item, workgroups done
+  void *d_queue = nullptr;  // grouped kernel's segment offsets and queues (see upload)
   int n_items = 0;
   bool dirty = true;
   void *graph = nullptr;
@@ -105,25 +106,63 @@ struct lth_batch {
 
   const char *tname() const { return dtype == 0 ? "bf16" : "f16"; }
 
+  // The work list: every output tile of every GEMM, cut into kXcd segments,
+  // one per XCD (workgroup w of the grouped launch runs on XCD w % kXcd and
+  // takes items of segment w % kXcd only).  A GEMM's tiles go to one XCD as
+  // long as that XCD stays within the average load, so the tiles that share
+  // an A or B panel fetch it into one L2; row strips (the tiles sharing an A
+  // panel) of GEMMs larger than that spill to the least-loaded XCDs.  Within
+  // a segment, items are ordered by decreasing cost, a GEMM's tiles adjacent.
   void upload() {
     if (!dirty) return;
-    std::vector<Item> items;
-    std::vector<int64_t> cost;
+    struct Strip {
+      int g, m0;
+      int64_t cost;
+    };
+    std::vector<Strip> strips;
+    std::vector<int64_t> gcost(descs.size(), 0);
     for (size_t g = 0; g < descs.size(); ++g) {
-      int c = cfg[g];
-      int bm = kTileM[c], bn = kTileN[c];
-      for (int m0 = 0; m0 < descs[g].M; m0 += bm)
-        for (int n0 = 0; n0 < descs[g].N; n0 += bn) {
-          items.push_back({(int)g, m0, n0, c});
-          cost.push_back((int64_t)bm * bn * descs[g].K);
-        }
+      const int c = cfg[g], bm = kTileM[c], bn = kTileN[c];
+      const int64_t tile = (int64_t)bm * bn * descs[g].K;
+      const int64_t ntn = (descs[g].N + bn - 1) / bn;
+      for (int m0 = 0; m0 < descs[g].M; m0 += bm) {
+        strips.push_back({(int)g, m0, ntn * tile});
+        gcost[g] += ntn * tile;
+      }
     }
-    std::vector<size_t> order(items.size());
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    int64_t total = 0;
+    for (auto c : gcost) total += c;
+    const double target = (double)total / kXcd;
+    // GEMMs by decreasing cost, strips of one GEMM in row order
+    std::stable_sort(strips.begin(), strips.end(), [&](const Strip &a, const Strip &b) {
+      return gcost[a.g] != gcost[b.g] ? gcost[a.g] > gcost[b.g] : a.g < b.g;
+    });
+    std::vector<double> load(kXcd, 0.0);
+    std::vector<std::vector<Item>> seg(kXcd);
+    std::vector<std::vector<int64_t>> segcost(kXcd);
+    int prev_g = -1, prev_x = 0;
+    for (const auto &st : strips) {
+      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      if (st.g == prev_g && load[prev_x] + st.cost <= target) x = prev_x;
+      load[x] += (double)st.cost;
+      prev_g = st.g;
+      prev_x = x;
+      const int c = cfg[st.g], bm = kTileM[c], bn = kTileN[c];
+      for (int n0 = 0; n0 < descs[st.g].N; n0 += bn) {
+        seg[x].push_back({st.g, st.m0, n0, c});
+        segcost[x].push_back((int64_t)bm * bn * descs[st.g].K);
+      }
+    }
     std::vector<Item> sorted;
-    for (auto i : order) sorted.push_back(items[i]);
+    std::vector<int> xoff(kXcd + 1, 0);
+    for (int x = 0; x < kXcd; ++x) {
+      std::vector<size_t> order(seg[x].size());
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](size_t a, size_t b) { return segcost[x][a] > segcost[x][b]; });
+      for (auto i : order) sorted.push_back(seg[x][i]);
+      xoff[x + 1] = (int)sorted.size();
+    }
     if (d_descs) lt_hip_free(d_descs);
     if (d_items) lt_hip_free(d_items);
     chk(lt_hip_malloc(&d_descs, descs.size() * sizeof(Desc)), "descs");
@@ -131,8 +170,12 @@ struct lth_batch {
     chk(lt_hip_memcpy(d_descs, descs.data(), descs.size() * sizeof(Desc), 0), "descs copy");
     if (!sorted.empty())
       chk(lt_hip_memcpy(d_items, sorted.data(), sorted.size() * sizeof(Item), 0), "items copy");
-    if (!d_queue) chk(lt_hip_malloc(&d_queue, 2 * sizeof(int)), "queue");
-    chk(lt_hip_memset(d_queue, 0, 2 * sizeof(int)), "queue reset");
+    // queue: [kXcd + 1] segment offsets, then kXcd segment counters and the
+    // count of workgroups done
+    if (!d_queue) chk(lt_hip_malloc(&d_queue, (2 * kXcd + 2) * sizeof(int)), "queue");
+    std::vector<int> q(2 * kXcd + 2, 0);
+    std::copy(xoff.begin(), xoff.end(), q.begin());
+    chk(lt_hip_memcpy(d_queue, q.data(), q.size() * sizeof(int), 0), "queue reset");
     n_items = (int)sorted.size();
     dirty = false;
     if (graph) {
@@ -172,9 +215,8 @@ struct lth_batch {
     unsigned lds = 0;
     void *fn = grouped_kernel(&lds);
     int g = std::max(1, std::min(grid, n_items));
-    void *a0 = d_descs, *a1 = d_items, *a3 = d_queue;
-    int a2 = n_items;
-    void *args[] = {&a0, &a1, &a2, &a3};
+    void *a0 = d_descs, *a1 = d_items, *a2 = d_queue;
+    void *args[] = {&a0, &a1, &a2};
     chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, lds, stream, args), "grouped launch");
   }
 

@@ -67,7 +67,8 @@ struct Knobs {
   int streamk = -1;            // "streamk=0|1": float contractions share K tiles
                                // evenly over the workgroups (-1 = auto: when
                                // the tiles fill the workgroup slots unevenly)
-  int overlap = -1;            // "overlap=0|1": run-ahead operand staging
+  int overlap = -1;            // "overlap=0|1": run-ahead operand staging (off
+                               // unless 1)
   int ksplit = -1;             // "ksplit=0|1": gathered contractions on 32x32
                                // tiles with K tiles split over the waves
                                // (-1 = auto: nests under 64 64x64 tiles)

@@ -1,22 +1,23 @@
 // Kernel library for batches of run-time-sized matmul nests (gfx950).
 //
 //   lt_gemm_<t>_<BM>x<BN>x<BK>(A, Bt, C, M, N, K)   one GEMM per launch
-//   lt_ggemm[_mid|_small]_<t>(descs, items, n_items, queue)
+//   lt_ggemm[_mid|_small]_<t>(descs, items, queue)
 //                                                    persistent grouped launch
 //   lt_cvt_<t>(src, dst, rows, cols, transpose)      float32 -> 16-bit operands
 //
 // Both launch forms run the software-pipelined implicit-GEMM tile of
 // igemm_tile.h (with its K-tail masking: K is any multiple of 8) on operands
 // A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
-// kernel walks a host-built work list of output tiles from many GEMMs,
-// ordered by decreasing cost, as a queue: workgroup w starts with item w and
-// then takes item grid + queue[0]++ (fetched while it computes the previous
-// one) until the list is empty, so the longest tiles start first and the
-// workgroups finish together.  The last workgroup
-// to leave (counted in queue[1]) resets both counters for the next launch,
-// so graph replays need no memset; launches of one work list must not
-// overlap.  Every item carries its own tile configuration, chosen per shape
-// bucket by the autotuner, so one launch mixes tile shapes.
+// kernel walks a host-built work list of output tiles from many GEMMs, cut
+// into one segment per XCD and ordered by decreasing cost within each: the
+// workgroups of XCD x take the items of segment x, first by index, then
+// from the segment's counter (fetched while they compute the previous
+// item), so the longest tiles start first, the workgroups finish together,
+// and tiles that share operand panels are fetched into one L2.  The last
+// workgroup to leave resets the counters for the next launch, so graph
+// replays need no memset; launches of one work list must not overlap.
+// Every item carries its own tile configuration, chosen per shape bucket by
+// the autotuner, so one launch mixes tile shapes.
 #include <hip/hip_runtime.h>
 
 #include "igemm_tile.h"
@@ -87,36 +88,48 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
         g, A, Bt, d.C, it.m0, it.n0, smem, (unsigned)d.M * d.K * 2, (unsigned)d.N * d.K * 2); \
     break;
 
+// queue: [0..8] the 8 segment offsets of the work list, [9..16] one item
+// counter per segment, [17] the count of workgroups done.  Workgroup w runs
+// on XCD w % 8 and takes items of that XCD's segment: first by index, then
+// from the segment's counter.  With fewer than 8 workgroups, workgroup w
+// runs segments w, w + grid, ... alone, in order.
 #define LT_GROUPED(T_, TN, CLASS, LIST)                                                      \
   extern "C" __global__ void __launch_bounds__(256)                                         \
       lt_ggemm##CLASS##_##TN(const LtGemmDesc* __restrict__ descs,                          \
-                             const LtWorkItem* __restrict__ items, int n_items,             \
-                             int* __restrict__ queue) {                                     \
+                             const LtWorkItem* __restrict__ items, int* __restrict__ queue) { \
     typedef T_ T;                                                                           \
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];                  \
     __shared__ int next_item;                                                               \
-    int i = blockIdx.x; /* the first items by index, the rest from the queue */             \
-    while (i < n_items) {                                                                   \
-      /* take the next item now: the atomic's latency hides behind this one */              \
-      int nxt = 0;                                                                          \
-      if (threadIdx.x == 0) nxt = (int)gridDim.x + atomicAdd(queue, 1);                     \
-      const LtWorkItem it = items[i];                                                       \
-      const LtGemmDesc d = descs[it.gemm];                                                  \
-      const T* A = (const T*)d.A;                                                           \
-      const T* Bt = (const T*)d.Bt;                                                         \
-      const LtMmGeo g = lt_mm_geo(d.M, d.N, d.K);                                           \
-      switch (it.cfg) { LIST(LT_CASE) default: break; }                                     \
-      __syncthreads(); /* LDS reuse; every lane has read next_item */                       \
-      if (threadIdx.x == 0) next_item = nxt;                                                \
-      __syncthreads();                                                                      \
-      i = next_item;                                                                        \
+    const int G = (int)gridDim.x;                                                           \
+    const bool shared_seg = G >= 8;                                                         \
+    for (int x = (int)blockIdx.x % 8; x < 8; x += shared_seg ? 8 : G) {                     \
+      const int beg = queue[x], end = queue[x + 1];                                         \
+      /* workgroups on this segment, and this one's rank among them */                      \
+      const int nx = shared_seg ? (G - x + 7) / 8 : 1;                                      \
+      int i = beg + (shared_seg ? (int)blockIdx.x / 8 : 0);                                 \
+      while (i < end) {                                                                     \
+        /* take the next item now: the atomic's latency hides behind this one */            \
+        int nxt = 0;                                                                        \
+        if (threadIdx.x == 0)                                                               \
+          nxt = shared_seg ? beg + nx + atomicAdd(queue + 9 + x, 1) : i + 1;                \
+        const LtWorkItem it = items[i];                                                     \
+        const LtGemmDesc d = descs[it.gemm];                                                \
+        const T* A = (const T*)d.A;                                                         \
+        const T* Bt = (const T*)d.Bt;                                                       \
+        const LtMmGeo g = lt_mm_geo(d.M, d.N, d.K);                                         \
+        switch (it.cfg) { LIST(LT_CASE) default: break; }                                   \
+        __syncthreads(); /* LDS reuse; every lane has read next_item */                     \
+        if (threadIdx.x == 0) next_item = nxt;                                              \
+        __syncthreads();                                                                    \
+        i = next_item;                                                                      \
+      }                                                                                     \
     }                                                                                       \
     if (threadIdx.x == 0) {                                                                 \
       __threadfence(); /* this workgroup's last take precedes its exit count */             \
-      if (atomicAdd(queue + 1, 1) == (int)gridDim.x - 1) {                                  \
+      if (atomicAdd(queue + 17, 1) == G - 1) {                                              \
         __threadfence(); /* every workgroup has taken its last item */                      \
-        atomicExch(queue, 0);                                                               \
-        atomicExch(queue + 1, 0);                                                           \
+        for (int x = 0; x < 8; ++x) atomicExch(queue + 9 + x, 0);                           \
+        atomicExch(queue + 17, 0);                                                          \
       }                                                                                     \
     }                                                                                       \
   }

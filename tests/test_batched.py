@@ -53,10 +53,11 @@ def test_grouped_and_graph_match_plain(bat, hip_dev, cfgs):
     want = [b.result(i) for i in range(len(dims))]
     for c in b.C:
         c.fill_zero()
-    # the work queue resets itself after each launch: launches in a row, with
-    # fewer workgroups than the GPU holds, as many, and more than there are
-    # items (the surplus finds the queue empty)
-    for grid in (97, 0, 100000, 0):
+    # the work queues reset themselves after each launch: launches in a row,
+    # with fewer workgroups than XCD segments (each runs whole segments
+    # alone), fewer than the GPU holds, as many, and more than there are
+    # items (the surplus finds the queues empty)
+    for grid in (97, 3, 0, 100000, 0):
         b.run_grouped(grid=grid)
         hip_dev.synchronize()
         for i in range(len(dims)):
