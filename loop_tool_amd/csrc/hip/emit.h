@@ -69,6 +69,9 @@ struct Knobs {
                                // the tiles fill the workgroup slots unevenly)
   int overlap = -1;            // "overlap=0|1": run-ahead operand staging (off
                                // unless 1)
+  int halo_direct = -1;        // "halo=direct|staged": the halo tile reads the
+                               // activations' float source instead of a staged
+                               // 16-bit copy (-1 = staged)
   int ksplit = -1;             // "ksplit=0|1": gathered contractions on 32x32
                                // tiles with K tiles split over the waves
                                // (-1 = auto: nests under 64 64x64 tiles)

@@ -586,6 +586,13 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     if (!fits(WGM, WGN) || (2 * BM + HB * HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
   }
   if (halo_forced && !HL) return false;
+  // halo=direct: the halo tile reads Q's float32 source itself (no staged
+  // copy of Q; its rows are rounded to 16 bits as they go to LDS)
+  const Operand &QOP = c.ops[c.qo];
+  const int64_t q_src_bytes =
+      df.numel(df.nodes.at(df.ir().inputs()[QOP.input_index]).vars) * 4;
+  const bool qdirect = HL && knobs.halo_direct == 1 && q_src_bytes < (1LL << 31);
+  if (knobs.halo_direct == 1 && HL && !qdirect) return false;
   const int lds = HL    ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
                         : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
@@ -609,7 +616,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   for (const auto &b : plan.buffers) scratch_no += b.kind == BufferSpec::SCRATCH;
   for (int o = 0; o < 2; ++o) {
     sbuf[o] = (int)plan.buffers.size();
-    int64_t words = ((st[o].total * EB + 3) / 4 + 7) / 8 * 8;
+    int64_t words = o == c.qo && qdirect ? 8 : ((st[o].total * EB + 3) / 4 + 7) / 8 * 8;
     auto nr = df.ir().inputs().at(c.ops[o].input_index);
     plan.buffers.push_back({BufferSpec::SCRATCH, scratch_no++, words, nr});
   }
@@ -650,13 +657,38 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     << lin_over(c.ostride, c.I) << ";\n  }\n"
     << "  __device__ long out_col(long j) const {\n"
     << decls(c.J, "    ") << decompose("j", c.J, "    ") << "    return "
-    << lin_over(c.ostride, c.J) << ";\n  }\n"
-    << "};\n";
+    << lin_over(c.ostride, c.J) << ";\n  }\n";
+  if (qdirect) {
+    // staged row r of a block -> source offset of its first k_in value;
+    // ok false on a zero halo row
+    const int in = SQ.inner_dim;
+    std::vector<int> rd;
+    for (int d : SQ.order)
+      if (d != in) rd.push_back(d);
+    k << "  static constexpr long QCS = " << QOP.strides[in] << "L, QKIN = " << QOP.sizes[in]
+      << "L;\n"
+      << "  __device__ long q_src(long r, bool& ok) const {\n    ok = true;\n    long src = 0;\n";
+    for (int q = (int)rd.size() - 1; q >= 0; --q) {
+      const int d = rd[q];
+      const std::string sv = "s" + std::to_string(d);
+      if (q > 0)
+        k << "    const long " << sv << " = r % " << SQ.ssize[d] << "L - " << SQ.padlo[d]
+          << "L; r /= " << SQ.ssize[d] << "L;\n";
+      else
+        k << "    const long " << sv << " = r - " << SQ.padlo[d] << "L;\n";
+      k << "    ok = ok && " << sv << " >= 0 && " << sv << " < " << QOP.sizes[d] << "L;\n"
+        << "    src += " << sv << " * " << QOP.strides[d] << "L;\n";
+    }
+    k << "    return src;\n  }\n";
+  }
+  k << "};\n";
   // ---- both staging passes in one launch: workgroups [0, nb[0]) stage
   // operand 0, the rest operand 1 ----
   int64_t nb[2];
-  for (int o = 0; o < 2; ++o)
+  for (int o = 0; o < 2; ++o) {
     nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt);
+    if (o == c.qo && qdirect) nb[o] = 0;  // the tile reads Q's source
+  }
   ASSERT(nb[0] + nb[1] + nzero < (1LL << 31)) << "staging grid too large";
   k << "static constexpr unsigned lt_stage_blocks[2] = {" << nb[0] << "u, " << nb[1] << "u};\n"
     << "extern \"C\" __global__ void __launch_bounds__(256) lt_stage0(const float* __restrict__ "
@@ -694,8 +726,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     s.grid[0] = (unsigned)(nb[0] + nb[1] + nzero);
     s.block[0] = 256;
     for (int o = 0; o < 2; ++o)
-      s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
-                 st[o].total * EB;
+      if (!(o == c.qo && qdirect))
+        s.bytes += df.numel(df.nodes.at(df.ir().inputs()[c.ops[o].input_index]).vars) * 4 +
+                   st[o].total * EB;
     plan.stages.push_back(s);
   }
   // eight waves: keep two workgroups per CU (four waves per SIMD, at most
@@ -708,7 +741,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
         : streamk ? " __attribute__((amdgpu_waves_per_eu(2)))"  // two 4-wave workgroups per CU
                   : "")
     << " lt_igemm0(const " << elt
-    << "* __restrict__ P, const " << elt << "* __restrict__ Q, float* __restrict__ out) {\n"
+    << "* __restrict__ P, const " << elt << "* __restrict__ Q, float* __restrict__ out"
+    << (qdirect ? ", const float* __restrict__ Qf" : "") << ") {\n"
     << "  __shared__ __attribute__((aligned(16))) unsigned char smem[" << lds << "];\n"
     << "  LtGeo g;\n"
     << decls(c.B, "  ") << decompose("(long)blockIdx.z", c.B, "  ")
@@ -741,6 +775,11 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     k << ind << "lt_igemm_tile_f32<" << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, " << i0
       << ", " << j0 << ", smem, " << SP.total * 4 << "u, " << SQ.total * 4 << "u);\n";
+  else if (HL && qdirect)
+    k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL << ", " << HB
+      << ", LtGeo, true>(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2
+      << "u, 0u, Qf, " << q_src_bytes << "u);\n";
   else if (HL)
     k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL << ", " << HB
@@ -761,13 +800,15 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   g.kernel = "lt_igemm0";
   g.kind = bf16 ? "contract_bf16" : f32 ? "contract_f32" : "contract_f16";
   g.buffers = {sbuf[c.po], sbuf[c.qo], out_buf};
+  if (qdirect) g.buffers.push_back(QOP.input_index);
   g.writes = {out_buf};
   g.grid[0] = (unsigned)(streamk ? SK : gx);
   g.grid[1] = (unsigned)(streamk ? 1 : gy);
   g.grid[2] = (unsigned)c.BD;
   g.block[0] = (unsigned)threads;
   g.flops = 2 * c.MD * c.ND * c.KD * c.BD;
-  g.bytes = (SP.total + SQ.total) * EB + c.MD * c.ND * c.BD * 4;
+  g.bytes = (qdirect ? SP.total * EB + q_src_bytes : (SP.total + SQ.total) * EB) +
+            c.MD * c.ND * c.BD * 4;
   plan.stages.push_back(g);
   plan.config["tile_m"] = BM;
   plan.config["tile_n"] = BN;
@@ -785,7 +826,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
                    : std::string())
        << (streamk ? ", stream-K over " + std::to_string(SK) + " workgroups" : std::string())
        << (HL ? ", Q halo of " + std::to_string(HL) + " rows per K block" +
-                    (HB == 1 ? " (one buffer)" : "")
+                    (HB == 1 ? " (one buffer)" : "") +
+                    (qdirect ? " read from the float source" : "")
               : std::string())
        << "; staged P " << SP.total << " and Q " << SQ.total
        << " elements\n";

@@ -23,6 +23,9 @@
 //   bk=N           K tile (32 or 64) of 16-bit contractions
 //   contract=gather  contractions gather their operands in the tile loader
 //                  instead of staging them
+//   halo=direct|staged  the halo tile reads the activations' float source
+//                  itself (rounding to 16 bits as rows go to LDS; only the
+//                  weights are staged) or a staged 16-bit copy (default)
 //   overlap=0|1    consecutive runs of a plan whose first stage stages
 //                  operands into scratch overlap that stage with the
 //                  previous run's main kernels (two scratch sets, a second
@@ -268,6 +271,9 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
         else bad();
       } else if (key == "streamk") {
         if (v == "0" || v == "1") k.streamk = v == "1";
+        else bad();
+      } else if (key == "halo") {
+        if (v == "direct" || v == "staged") k.halo_direct = v == "direct";
         else bad();
       } else if (key == "overlap") {
         if (v == "0" || v == "1") k.overlap = v == "1";

@@ -394,13 +394,25 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
 // two-tiles-ahead register pipeline and double-buffered LDS as in
 // lt_igemm_tile_pipe.  HL (a host-computed bound over all tiles) must cover
 // (q_row(j_last) - q_row(j0) + max tap shift) / BK + 1.
+//
+// QF: the halo is read from Q's float32 source instead of a staged copy.
+// Staged row r of a block (the position of one BK-vector) comes from
+// g.q_src(r, ok) -- the source offset of its first k_in, ok false on a zero
+// halo row -- and the BK values of block kb at k_in = kb * BK + kr, kr < BK,
+// G::QCS elements apart (zero past G::QKIN).  Each lane loads its 8-value
+// chunks with dword buffer loads while the block's first tap runs, and
+// rounds them to T (as the staging launch does) when it stores the halo: the
+// same operand values in the same order as the staged form.  Qf and qfbytes
+// then describe the source (under 2^31 bytes); Q and qbytes are unused.
 template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, int HB,
-          class G>
+          class G, bool QF = false>
 __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restrict__ P,
                                                    const T* __restrict__ Q,
                                                    float* __restrict__ out, long i0, long j0,
                                                    unsigned char* smem, unsigned pbytes,
-                                                   unsigned qbytes) {
+                                                   unsigned qbytes,
+                                                   const float* __restrict__ Qf = nullptr,
+                                                   unsigned qfbytes = 0) {
   typedef typename lt_ig_vec8<T>::type V;
   constexpr int NT = 64 * WGM * WGN, LDR = BK + 8;
   constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
@@ -449,23 +461,59 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
 #pragma unroll
   for (int h = 0; h < HV; ++h) {
     const int e = tid + h * NT;
-    hv[h] = e < HC ? (unsigned)((qb0 + (long)(e / CPR) * BK + (e % CPR) * 8) * 2) : 0x80000000u;
+    if constexpr (QF) {
+      // the source offset (bytes) of the chunk's first value in block 0
+      bool ok = false;
+      const long src = e < HC ? g.q_src(qb0 / BK + e / CPR, ok) : 0;
+      hv[h] = ok ? (unsigned)((src + (long)((e % CPR) * 8) * G::QCS) * 4) : 0x80000000u;
+    } else {
+      hv[h] = e < HC ? (unsigned)((qb0 + (long)(e / CPR) * BK + (e % CPR) * 8) * 2)
+                     : 0x80000000u;
+    }
   }
-  lt_ig_u32x4 hr[HV];
+  lt_ig_u32x4 hr[QF ? 1 : HV];
+  float hf[QF ? HV : 1][8];
+  const auto rsQf = __builtin_amdgcn_make_buffer_rsrc((void*)Qf, (short)0, (int)qfbytes,
+                                                      0x00020000);
   // the block offset goes into the per-lane offset, which the buffer range
-  // check covers: a halo tail past the end of Q reads zeros
+  // check covers: a halo tail past the end of Q (or a zero halo row of the
+  // source, offset 2^31) reads zeros
   auto hload = [&](int kb) {
-    const unsigned bo = (unsigned)(g.q_koff(kb * NTAP) * 2);
+    if constexpr (QF) {
 #pragma unroll
-    for (int h = 0; h < HV; ++h)
-      hr[h] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)(hv[h] + bo), 0, 0);
+      for (int h = 0; h < HV; ++h) {
+        const int c0 = kb * BK + ((tid + h * NT) % CPR) * 8;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const unsigned off = (G::QKIN % BK == 0 || c0 + v < G::QKIN)
+                                   ? hv[h] + (unsigned)((long)(kb * BK + v) * G::QCS * 4)
+                                   : 0x80000000u;
+          hf[h][v] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(rsQf, (int)off, 0, 0));
+        }
+      }
+    } else {
+      const unsigned bo = (unsigned)(g.q_koff(kb * NTAP) * 2);
+#pragma unroll
+      for (int h = 0; h < HV; ++h)
+        hr[h] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)(hv[h] + bo), 0, 0);
+    }
   };
   auto hstore = [&](int buf) {
 #pragma unroll
     for (int h = 0; h < HV; ++h) {
       const int e = tid + h * NT;
-      if (HC % NT == 0 || e < HC)
-        *(lt_ig_u32x4*)(sH + buf * HL * LDR + (e / CPR) * LDR + (e % CPR) * 8) = hr[h];
+      if (HC % NT == 0 || e < HC) {
+        lt_ig_u32x4* dst = (lt_ig_u32x4*)(sH + buf * HL * LDR + (e / CPR) * LDR + (e % CPR) * 8);
+        if constexpr (QF) {
+          V x;
+#pragma unroll
+          for (int v = 0; v < 8; ++v) x[v] = (T)hf[h][v];
+          *dst = __builtin_bit_cast(lt_ig_u32x4, x);
+        } else {
+          *dst = hr[h];
+        }
+      }
     }
   };
   lt_ig_u32x4 r0p[VP], r1p[VP];

@@ -75,6 +75,29 @@ def test_conv2d_bf16_benchmark_plan(lt, hip_dev):
     assert np.array_equal(dy.download((N, C, H, H)).view(np.uint32), y1.view(np.uint32))
 
 
+def test_conv2d_bf16_direct_halo_bit_identical(lt, hip_dev):
+    """The benchmark conv with the halo read from the float source
+    (halo=direct: only the weights are staged) gives the bits of the staged
+    halo plan."""
+    N, C, H = 32, 128, 56
+    out, shapes = P.conv2d(lt, N, C, C, H, H)
+    std = 1.0 / np.sqrt(C * 9)
+    X, Wt = P.normal(4, shapes[0], std), P.normal(5, shapes[1], std)
+    dx = hip_dev.DeviceArray.from_numpy(X)
+    dw = hip_dev.DeviceArray.from_numpy(Wt)
+    ys = []
+    for knobs in ("", "halo=direct"):
+        t = out.loop_tree
+        if knobs:
+            t = t.annotate(t.roots[0], knobs)
+        cc = lt.compile(t, "hip_bf16")
+        assert ("from the float source" in cc.dump().splitlines()[0]) == bool(knobs)
+        dy = hip_dev.DeviceArray(N * C * H * H)
+        cc.run([dx.ptr, dw.ptr, dy.ptr])
+        ys.append(dy.download())
+    assert np.array_equal(ys[0].view(np.uint32), ys[1].view(np.uint32))
+
+
 def test_softmax_8192_every_row(lt, hip_dev):
     R = Cn = 8192
     out, shapes = P.softmax(lt, R, Cn)

@@ -214,8 +214,9 @@ HALO = [
                                   "tile=128x512"])
 def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     """The halo form of the staged tile (the Q rows of all window taps of a
-    channel block loaded once into LDS, in two buffers or one) computes the
-    same K order as the
+    channel block loaded once into LDS, in two buffers or one, from a staged
+    16-bit copy or straight from the float source) computes the same K order
+    as the
     pipelined tile over the same k-blocked staging (blocks outer, taps
     inner): the two results must be bit-identical, and within float32
     accumulation error of float64 over the bf16-rounded operands.  Covers
@@ -226,12 +227,14 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
     ys = []
-    for knobs in ("igemm=pipe stage=blocked kb=outer", "igemm=halo", "igemm=halo1"):
+    for knobs in ("igemm=pipe stage=blocked kb=outer", "igemm=halo", "igemm=halo1",
+                  "igemm=halo halo=direct", "igemm=halo1 halo=direct"):
         t = out.loop_tree
         t = t.annotate(t.roots[0], (knobs + " " + tile).strip())
         cc = lt.compile(t, "hip_bf16")
         assert cc.string_property("kinds") == "stage,contract_bf16"
         assert ("Q halo" in cc.dump().splitlines()[0]) == knobs.startswith("igemm=halo")
+        assert ("from the float source" in cc.dump().splitlines()[0]) == ("direct" in knobs)
         y = np.zeros(t.io_numel()[len(inputs)], np.float32)
         cc.run(inputs + [y])
         ys.append(y)
