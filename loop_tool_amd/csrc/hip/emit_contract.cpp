@@ -288,8 +288,10 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   k << "extern \"C\" __global__ void __launch_bounds__(" << NT << ") lt_gemm0(";
   for (int i = 0; i < df.num_inputs; ++i) k << "const float* __restrict__ in" << i << ", ";
   k << "float* __restrict__ out) {\n";
-  k << "  __shared__ " << elt << " sP[2][" << BM << "][" << BK + PAD << "];\n";
-  k << "  __shared__ " << elt << " sQ[2][" << BN << "][" << BK + PAD << "];\n";
+  const int64_t NKT = (KD + BK - 1) / BK;
+  const int NBUF = NKT > 1 ? 2 : 1;  // LDS buffers: one when a single K tile
+  k << "  __shared__ " << elt << " sP[" << NBUF << "][" << BM << "][" << BK + PAD << "];\n";
+  k << "  __shared__ " << elt << " sQ[" << NBUF << "][" << BN << "][" << BK + PAD << "];\n";
   k << "  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;\n";
   if (ksplit)  // every wave covers the whole tile, over its quarter of K
     k << "  const int wm = 0, wn = 0, kw = wave * " << BKW << ";\n"
@@ -444,7 +446,6 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   k << "  lt_f32x16 acc[" << TM << "][" << TN << "];\n";
   k << "  #pragma unroll\n  for (int a = 0; a < " << TM << "; ++a)\n  #pragma unroll\n"
        "  for (int b = 0; b < " << TN << "; ++b) acc[a][b] = (lt_f32x16){};\n";
-  const int64_t NKT = (KD + BK - 1) / BK;
   k << "  long k0 = 0;\n  int buf = 0;\n";
   k << loadP << loadQ << storeP << storeQ << "  __syncthreads();\n";
   k << "  for (long kt = 0; kt < " << NKT << "; ++kt) {\n";
