@@ -59,8 +59,9 @@ struct Knobs {
                                // the software-pipelined one
   int waves = 0;               // "waves=2|4|8": waves per workgroup of the
                                // pipelined 16-bit tile (0 = auto)
-  int lds_swz = -1;            // "lds=swz|pad": XOR-permuted or padded LDS rows
-                               // of the pipelined tile (-1 = auto)
+  int lds_swz = -1;            // "lds=pad|swz|dl" (0, 1, 2): padded or XOR-permuted
+                               // LDS rows of the register-staged pipelined tile,
+                               // or the direct-to-LDS tile (-1 = auto: dl)
   int stage_blocked = -1;      // "stage=blocked|flat": k-blocked operand staging
                                // (-1 = auto: blocked when k_in spans several K tiles)
   int kb_outer = -1;           // "kb=outer|inner": order of the K blocks

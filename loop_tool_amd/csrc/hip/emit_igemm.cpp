@@ -511,6 +511,12 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
       }
     }
     if (knobs.bk > 0) BK = knobs.bk;
+    // the direct-to-LDS tile (the default for 16-bit operands) runs K tiles
+    // of 64: plain contractions (no outer-K positions, so no halo form) whose
+    // k_in spans several such tiles
+    if (knobs.bk <= 0 && !f32 && ko_n == 1 && (KI % 64 == 0 || KI >= 256) &&
+        (knobs.lds_swz == -1 || knobs.lds_swz == 2) && !knobs.igemm_plain)
+      BK = 64;
   }
   if (s_bk == 32 || s_bk == 64) BK = (int)s_bk;
   if (BK != 32 && BK != 64) return false;
@@ -586,6 +592,9 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     if (!fits(WGM, WGN) || (2 * BM + HB * HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
   }
   if (halo_forced && !HL) return false;
+  // direct-to-LDS tile: 16-bit pipelined tiles outside the halo form, unless
+  // lds=swz|pad asks for the register-staged one
+  const bool dl = pipe && !f32 && !HL && (knobs.lds_swz == -1 || knobs.lds_swz == 2);
   // halo=direct: the halo tile reads Q's float32 source itself (no staged
   // copy of Q; its rows are rounded to 16 bits as they go to LDS)
   const Operand &QOP = c.ops[c.qo];
@@ -595,7 +604,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   if (knobs.halo_direct == 1 && HL && !qdirect) return false;
   const int lds = HL    ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
-                        : 2 * (BM + BN) * (swz ? BK : BK + 8) * 2;
+                        : 2 * (BM + BN) * (swz || dl ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;
   const int64_t gx = (c.ND + BN - 1) / BN, gy = (c.MD + BM - 1) / BM;
   ASSERT(gx < (1LL << 31) && gy < 65536 && c.BD < 65536) << "contraction grid too large";
@@ -785,6 +794,10 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL << ", " << HB
       << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2 << "u, "
       << SQ.total * 2 << "u);\n";
+  else if (dl)
+    k << ind << "lt_igemm_tile_dl<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, " << i0
+      << ", " << j0 << ", smem, " << SP.total * 2 << "u, " << SQ.total * 2 << "u);\n";
   else if (pipe)
     k << ind << "lt_igemm_tile_pipe<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", "
@@ -814,13 +827,14 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   plan.config["tile_n"] = BN;
   plan.config["bk"] = BK;
   plan.config["waves"] = pipe ? WGM * WGN : 4;
+  plan.config["dl"] = dl ? 1 : 0;
   plan.source = k.str();
   std::stringstream desc;
   desc << "[hip] contraction on MFMA (" << (bf16 ? "bf16" : f32 ? "f32" : "f16")
        << " staged operands, f32 accumulate): D[" << c.BD << "][" << c.MD << "][" << c.ND
        << "] = P x Q over K=" << ko_n << "x" << (blocked ? KIT : KI) << (KIP != KI && !blocked ? " (padded to " + std::to_string(KIP) + ")" : std::string()) << ", tile " << BM << "x" << BN << "x"
        << BK << (pipe ? " pipelined, " + std::to_string(WGM * WGN) + " waves" +
-                            (swz ? ", swizzled LDS" : "")
+                            (dl ? ", direct-to-LDS loads" : swz ? ", swizzled LDS" : "")
                       : std::string())
        << (blocked ? std::string(", k-blocked staging (kb ") + (kb_first ? "outer)" : "inner)")
                    : std::string())

@@ -42,8 +42,10 @@
 //   igemm=plain    staged 16-bit contractions use the single-stage tile
 //                  (default: the software-pipelined tile where it applies)
 //   waves=2|4|8    waves per workgroup of the pipelined tile
-//   lds=swz|pad    LDS rows of the pipelined tile: XOR-permuted 16-byte
-//                  chunks (no padding) or rows padded by 16 bytes
+//   lds=dl|swz|pad LDS of the pipelined 16-bit tile: filled by direct-to-LDS
+//                  loads into XOR-permuted rows (dl, the default outside the
+//                  halo form), or through registers into XOR-permuted (swz)
+//                  or 16-byte padded (pad) rows
 //   stage=blocked|flat  16-bit contraction operands staged k-blocked
 //                  ([K/BK][...][BK]) or with the whole k_in dim innermost
 //   kb=outer|inner order of the K blocks among the outer-K positions
@@ -289,6 +291,7 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
         else bad();
       } else if (key == "lds") {
         if (v == "swz" || v == "pad") k.lds_swz = v == "swz";
+        else if (v == "dl") k.lds_swz = 2;
         else bad();
       } else if (key == "stage") {
         if (v == "blocked" || v == "flat") k.stage_blocked = v == "blocked";

@@ -377,6 +377,141 @@ __device__ __forceinline__ void lt_igemm_tile_pipe(const G& g, const T* __restri
   lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
 }
 
+// Direct-to-LDS form of the pipelined tile: the same operands, contract and
+// bounds handling as lt_igemm_tile_pipe (SWZ layout), but the global loads
+// write LDS themselves (buffer_load_dwordx4 ... lds: lane l of a wave's load
+// lands at the wave's LDS base + 16 l), so no registers hold tiles in flight
+// and no LDS stores are issued.  Lane l of round v therefore fetches the
+// 16-byte chunk that belongs at slot s = v * NT + tid of the XOR-permuted
+// tile: row s / CPR, logical chunk (s % CPR) ^ ((row / RP) % CPR).
+//  * Two LDS stages.  Tile kt + 1 is fetched into the other stage while tile
+//    kt is multiplied: at the top of each K tile a wave waits for its own
+//    loads (vmcnt 0), and one barrier makes everyone's visible and frees the
+//    stage that the next loads overwrite.
+//  * The freed registers pay for BK = 64: half the barriers of BK = 32 per K.
+//  * sched_group_barrier interleaves, per k16 step, each MFMA with one
+//    fragment read of the next step, and the next tile's loads with the
+//    first step's MFMAs.
+// LDS: 2 * (BM + BN) * BK 16-bit elements.  KTAIL as in lt_igemm_tile_pipe:
+// chunks past KI in the last K tile read the buffer's out-of-range zero.
+template <int BM, int BN, int BK>
+__host__ __device__ constexpr int lt_igemm_dl_lds_bytes() {
+  return 2 * (BM + BN) * BK * 2;
+}
+
+typedef __attribute__((address_space(3))) void lt_ig_lds_void;
+
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool KTAIL = false,
+          class G>
+__device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict__ P,
+                                                 const T* __restrict__ Q,
+                                                 float* __restrict__ out, long i0, long j0,
+                                                 unsigned char* smem, unsigned pbytes,
+                                                 unsigned qbytes) {
+  typedef typename lt_ig_vec8<T>::type V;
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+  constexpr int CPR = BK / 8, KK = BK / 16, RP = 128 / BK;
+  constexpr int VP = BM * CPR / NT, VQ = BN * CPR / NT;
+  constexpr int SP = BM * BK, SQ = BN * BK;  // elements of one stage
+  static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
+  static_assert(VP >= 1 && VQ >= 1 && BM * CPR % NT == 0 && BN * CPR % NT == 0,
+                "tile too small for the workgroup");
+  static_assert(KK % 2 == 0, "BK must be a multiple of 32");
+  const int KIT = KTAIL ? (g.KI + BK - 1) / BK : g.KI / BK;
+  const int KT = g.KO * KIT;
+  T* sP = (T*)smem;         // [2][BM * BK]
+  T* sQ = sP + 2 * SP;      // [2][BN * BK]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
+  const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
+  // byte offset of this lane's chunk in each load round (0x80000000: a row
+  // outside the problem, read as zeros) and the chunk's first k
+  unsigned pv[VP], qv[VQ];
+  int pk[VP], qk[VQ];
+#pragma unroll
+  for (int v = 0; v < VP; ++v) {
+    const int s = v * NT + tid, r = s / CPR, c = (s % CPR) ^ ((r / RP) % CPR);
+    bool ok;
+    const long o = g.p_row(i0 + r, ok);
+    pv[v] = (FULL || ok) ? (unsigned)((o + c * 8) * 2) : 0x80000000u;
+    pk[v] = c * 8;
+  }
+#pragma unroll
+  for (int v = 0; v < VQ; ++v) {
+    const int s = v * NT + tid, r = s / CPR, c = (s % CPR) ^ ((r / RP) % CPR);
+    bool ok;
+    const long o = g.q_row(j0 + r, ok);
+    qv[v] = (FULL || ok) ? (unsigned)((o + c * 8) * 2) : 0x80000000u;
+    qk[v] = c * 8;
+  }
+  auto load = [&](int kt, int buf) {
+    const int ko = kt / KIT, kb = (kt % KIT) * BK;
+    const int po = (int)((g.p_koff(ko) + kb) * 2), qo = (int)((g.q_koff(ko) + kb) * 2);
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsP, (lt_ig_lds_void*)(sP + buf * SP + (v * NT + wave * 64) * 8), 16,
+          (int)(KTAIL && kb + pk[v] >= g.KI ? 0x80000000u : pv[v]), po, 0, 0);
+#pragma unroll
+    for (int v = 0; v < VQ; ++v)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsQ, (lt_ig_lds_void*)(sQ + buf * SQ + (v * NT + wave * 64) * 8), 16,
+          (int)(KTAIL && kb + qk[v] >= g.KI ? 0x80000000u : qv[v]), qo, 0, 0);
+  };
+  const int fr = lane & 31, fh = lane >> 5;
+  V fa[2][TM], fb[2][TN];
+  auto frags = [&](int buf, int kk, int s) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[s][a] = *(const V*)(sP + buf * SP +
+                             lt_ig_lds_off<BK, true>(wm * WM + a * 32 + fr, 2 * kk + fh));
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+      fb[s][b] = *(const V*)(sQ + buf * SQ +
+                             lt_ig_lds_off<BK, true>(wn * WN + b * 32 + fr, 2 * kk + fh));
+  };
+  lt_ig_f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  // vmcnt(0) with lgkmcnt and expcnt left alone (gfx9 s_waitcnt encoding)
+  constexpr int kWaitVm0 = (0xF << 8) | (0x7 << 4);
+  auto step = [&](int kt, int B) {
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's loads of tile kt have landed
+    __syncthreads();                       // everyone's have; stage B^1 is free
+    const bool more = kt + 1 < KT;
+    frags(B, 0, 0);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const int s = kk & 1;
+      if (kk == 0 && more) load(kt + 1, B ^ 1);
+      if (kk + 1 < KK) frags(B, kk + 1, s ^ 1);
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
+      constexpr int NM = TM * TN, NR = TM + TN, NL = VP + VQ;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);  // MFMA
+        if (kk + 1 < KK && i < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);  // LDS read
+        if (kk == 0 && i < NL) __builtin_amdgcn_sched_group_barrier(0x0020, 1, 0);  // load
+      }
+    }
+  };
+  load(0, 0);
+  int kt = 0;
+  for (; kt + 1 < KT; kt += 2) {
+    step(kt, 0);
+    step(kt + 1, 1);
+  }
+  if (kt < KT) step(kt, 0);
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+}
+
 // Halo form of the pipelined tile, for a k-blocked staged Q whose outer-K
 // positions within a K block ("taps", g.NTAP of them) only shift the row
 // offset: Q(j, kb, tap, kr) = q_row(j) + q_koff(kb * NTAP + tap) + kr with
