@@ -49,9 +49,9 @@ const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128, 128};
 const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256, 128};
 const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32, 32};
 
-// LDS of the pipelined tile of configuration c (igemm_tile.h,
-// lt_igemm_lds_bytes: two stages of both operand tiles, rows padded by 16 B)
-int lds_bytes(int c) { return 2 * (kTileM[c] + kTileN[c]) * (kTileK[c] + 8) * 2; }
+// LDS of the direct-to-LDS tile of configuration c (igemm_tile.h,
+// lt_igemm_dl_lds_bytes: two stages of both operand tiles, unpadded rows)
+int lds_bytes(int c) { return 2 * (kTileM[c] + kTileN[c]) * kTileK[c] * 2; }
 
 void chk(int rc, const char *what) {
   ASSERT(rc == 0) << what << ": " << lt_hip_last_error();

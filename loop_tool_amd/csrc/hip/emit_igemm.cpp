@@ -590,6 +590,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     WGM = 2;
     WGN = knobs.waves != 4 && fits(2, 4) ? 4 : 2;
     if (!fits(WGM, WGN) || (2 * BM + HB * HL) * (BK + 8) * 2 > 160 * 1024) HL = 0;
+    if (HL) swz = false;  // the register-staged halo tile pads its LDS rows
   }
   if (halo_forced && !HL) return false;
   // direct-to-LDS tile: 16-bit pipelined tiles outside the halo form, unless
@@ -602,7 +603,15 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
       df.numel(df.nodes.at(df.ir().inputs()[QOP.input_index]).vars) * 4;
   const bool qdirect = HL && knobs.halo_direct == 1 && q_src_bytes < (1LL << 31);
   if (knobs.halo_direct == 1 && HL && !qdirect) return false;
-  const int lds = HL    ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
+  // direct-to-LDS halo tile (lds=dl): P stages and halo buffers filled by
+  // loads that write LDS; the halo rounded up to whole load rounds
+  const int64_t hthreads = 64 * WGM * WGN, hcpr = BK / 8;
+  const int64_t HLP = HL * hcpr >= hthreads ? HL : hthreads / hcpr;
+  const bool hdl = HL && !qdirect && knobs.lds_swz == 2 &&
+                   2 * (BM + HLP) * BK * 2 <= 160 * 1024 &&
+                   (HL * hcpr + hthreads - 1) / hthreads <= 15;
+  const int lds = hdl   ? (int)(2 * (BM + HLP) * BK * 2)
+                  : HL  ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
                         : 2 * (BM + BN) * (swz || dl ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;
@@ -784,6 +793,11 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
     k << ind << "lt_igemm_tile_f32<" << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ">(g, P, Q, out, " << i0
       << ", " << j0 << ", smem, " << SP.total * 4 << "u, " << SQ.total * 4 << "u);\n";
+  else if (hdl)
+    k << ind << "lt_igemm_tile_halo_dl<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
+      << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL
+      << ">(g, P, Q, out, " << i0 << ", " << j0 << ", smem, " << SP.total * 2 << "u, "
+      << SQ.total * 2 << "u);\n";
   else if (HL && qdirect)
     k << ind << "lt_igemm_tile_halo<" << elt << ", " << BM << ", " << BN << ", " << BK << ", "
       << (full ? "true" : "false") << ", " << WGM << ", " << WGN << ", " << HL << ", " << HB

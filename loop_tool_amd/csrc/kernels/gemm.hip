@@ -5,7 +5,7 @@
 //                                                    persistent grouped launch
 //   lt_cvt_<t>(src, dst, rows, cols, transpose)      float32 -> 16-bit operands
 //
-// Both launch forms run the software-pipelined implicit-GEMM tile of
+// Both launch forms run the direct-to-LDS pipelined implicit-GEMM tile of
 // igemm_tile.h (with its K-tail masking: K is any multiple of 8) on operands
 // A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
 // kernel walks a host-built work list of output tiles from many GEMMs, cut
@@ -76,15 +76,15 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
   extern "C" __global__ void __launch_bounds__(256)                                         \
       lt_gemm_##TN##_##BM##x##BN##x##BK(const T* __restrict__ A, const T* __restrict__ Bt,  \
                                         float* __restrict__ C, int M, int N, int K) {       \
-    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_lds_bytes<BM, BN, BK>()]; \
-    lt_igemm_tile_pipe<T, BM, BN, BK, false, 2, 2, false, true>(                            \
+    __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_dl_lds_bytes<BM, BN, BK>()]; \
+    lt_igemm_tile_dl<T, BM, BN, BK, false, 2, 2, true>(                                     \
         lt_mm_geo(M, N, K), A, Bt, C, (long)blockIdx.y * BM, (long)blockIdx.x * BN, smem,  \
         (unsigned)M * K * 2, (unsigned)N * K * 2);                                          \
   }
 
 #define LT_CASE(i, BM, BN, BK)                                                               \
   case i:                                                                                    \
-    lt_igemm_tile_pipe<T, BM, BN, BK, false, 2, 2, false, true>(                            \
+    lt_igemm_tile_dl<T, BM, BN, BK, false, 2, 2, true>(                                     \
         g, A, Bt, d.C, it.m0, it.n0, smem, (unsigned)d.M * d.K * 2, (unsigned)d.N * d.K * 2); \
     break;
 

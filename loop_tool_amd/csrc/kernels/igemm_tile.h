@@ -481,7 +481,10 @@ __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict
   constexpr int kWaitVm0 = (0xF << 8) | (0x7 << 4);
   auto step = [&](int kt, int B) {
     __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's loads of tile kt have landed
-    __syncthreads();                       // everyone's have; stage B^1 is free
+    // everyone's have, and every wave is past its reads of stage B^1 (each
+    // read fed an MFMA already issued); a bare barrier, as a fence here would
+    // also wait for the LDS loads still in flight
+    __builtin_amdgcn_s_barrier();
     const bool more = kt + 1 < KT;
     frags(B, 0, 0);
 #pragma unroll
@@ -504,6 +507,7 @@ __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict
   };
   load(0, 0);
   int kt = 0;
+#pragma unroll 1  // a K loop unrolled whole would hold every K tile's addresses
   for (; kt + 1 < KT; kt += 2) {
     step(kt, 0);
     step(kt + 1, 1);
@@ -742,6 +746,166 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   } else {
     step(kt, 0, r1p, r0p, false, false);
   }
+  lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
+}
+
+// Direct-to-LDS form of the halo tile (same operands, geometry and HL bound
+// as lt_igemm_tile_halo): P tiles go through two XOR-permuted LDS stages as in
+// lt_igemm_tile_dl, and the halo of each K block through two XOR-permuted
+// halo buffers, both filled by buffer loads that write LDS.  Halo rows are
+// rounded up to whole load rounds (HV rounds of NT chunks; the extra chunks
+// read the out-of-range zero), so every wave issues the same loads and the
+// waits can count them.  Order of loads: P of tile kt + 1 at the top of tile
+// kt, then, at a block's first tap, the next block's halo; the top of the
+// second tap waits for everything but that halo (vmcnt HV), the other taps
+// for everything.  A tap's Q fragments are halo rows shifted by the tap's
+// row offset; their chunk permutation follows the shifted row.  A halo of
+// at least NT chunks is loaded in HV rounds of NT chunks whose last round
+// ends at the halo's last chunk (it reloads a few chunks of the round before:
+// two waves write the same bytes, before the barrier that precedes any read),
+// so the buffers hold exactly HL rows; a smaller halo takes one round, its
+// chunks past HL reading the out-of-range zero into padding rows.
+// LDS: 2 * (BM + HLP) * BK 16-bit elements, HLP = max(HL, NT / CPR).
+template <int BM, int BK, int HL, int NT>
+__host__ __device__ constexpr int lt_igemm_halo_dl_lds_bytes() {
+  return 2 * (BM + (HL * (BK / 8) >= NT ? HL : NT / (BK / 8))) * BK * 2;
+}
+
+template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, class G>
+__device__ __forceinline__ void lt_igemm_tile_halo_dl(const G& g, const T* __restrict__ P,
+                                                      const T* __restrict__ Q,
+                                                      float* __restrict__ out, long i0, long j0,
+                                                      unsigned char* smem, unsigned pbytes,
+                                                      unsigned qbytes) {
+  typedef typename lt_ig_vec8<T>::type V;
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 32, TN = WN / 32;
+  constexpr int CPR = BK / 8, KK = BK / 16, RP = 128 / BK;
+  constexpr int VP = BM * CPR / NT;
+  constexpr int HC = HL * CPR, HV = (HC + NT - 1) / NT, HLP = HC >= NT ? HL : NT / CPR;
+  constexpr int SP = BM * BK, SH = HLP * BK;  // elements of a P stage, a halo buffer
+  static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
+  static_assert(VP >= 1 && BM * CPR % NT == 0, "tile too small for the workgroup");
+  static_assert(KK % 2 == 0, "BK must be a multiple of 32");
+  static_assert(HV <= 15, "halo loads beyond the vmcnt range");
+  constexpr int NTAP = G::NTAP;
+  static_assert(NTAP >= 2, "the halo form needs at least two taps per K block");
+  const int KT = g.KO, KB = KT / NTAP;
+  T* sP = (T*)smem;     // [2][BM * BK]
+  T* sH = sP + 2 * SP;  // [2][HLP * BK]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WGN, wn = wave % WGN;
+  const auto rsP = __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)pbytes, 0x00020000);
+  const auto rsQ = __builtin_amdgcn_make_buffer_rsrc((void*)Q, (short)0, (int)qbytes, 0x00020000);
+  unsigned pv[VP];
+#pragma unroll
+  for (int v = 0; v < VP; ++v) {
+    const int s = v * NT + tid, r = s / CPR, c = (s % CPR) ^ ((r / RP) % CPR);
+    bool ok;
+    const long o = g.p_row(i0 + r, ok);
+    pv[v] = (FULL || ok) ? (unsigned)((o + c * 8) * 2) : 0x80000000u;
+  }
+  // the tile's first Q row; each fragment row's halo row
+  bool ok0;
+  const long qb0 = g.q_row(j0, ok0);
+  const int fr = lane & 31, fh = lane >> 5;
+  int lq[TN];
+#pragma unroll
+  for (int b = 0; b < TN; ++b) {
+    long j = j0 + wn * WN + b * 32 + fr;
+    if (!FULL && j >= g.ND) j = g.ND - 1;  // rows past the problem: results not stored
+    bool ok;
+    lq[b] = (int)((g.q_row(j, ok) - qb0) / BK);
+  }
+  // halo chunks of this lane: slot s = hbase(h) + tid is halo row s / CPR
+  auto hbase = [](int h) { return HC >= NT ? (h * NT < HC - NT ? h * NT : HC - NT) : 0; };
+  unsigned hv[HV];
+#pragma unroll
+  for (int h = 0; h < HV; ++h) {
+    const int s = hbase(h) + tid, r = s / CPR, c = (s % CPR) ^ ((r / RP) % CPR);
+    hv[h] = s < HC ? (unsigned)((qb0 + (long)r * BK + c * 8) * 2) : 0x80000000u;
+  }
+  // the block offset goes into the per-lane offset, which the buffer range
+  // check covers: a halo tail past the end of Q reads zeros
+  auto hload = [&](int kb, int buf) {
+    const unsigned bo = (unsigned)(g.q_koff(kb * NTAP) * 2);
+#pragma unroll
+    for (int h = 0; h < HV; ++h)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsQ, (lt_ig_lds_void*)(sH + buf * SH + (hbase(h) + wave * 64) * 8), 16,
+          (int)(hv[h] + bo), 0, 0, 0);
+  };
+  auto load = [&](int kt, int buf) {
+    const int po = (int)(g.p_koff(kt) * 2);
+#pragma unroll
+    for (int v = 0; v < VP; ++v)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rsP, (lt_ig_lds_void*)(sP + buf * SP + (v * NT + wave * 64) * 8), 16, (int)pv[v], po,
+          0, 0);
+  };
+  // halo row shift of tile kt's tap
+  auto shift = [&](int kt) {
+    const int kb = kt / NTAP;
+    return (int)((g.q_koff(kt) - g.q_koff(kb * NTAP)) / BK);
+  };
+  V fa[2][TM], fb[2][TN];
+  auto frags = [&](int B, int hb, int sh, int kk, int s) {
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+      fa[s][a] = *(const V*)(sP + B * SP +
+                             lt_ig_lds_off<BK, true>(wm * WM + a * 32 + fr, 2 * kk + fh));
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+      fb[s][b] = *(const V*)(sH + hb * SH + lt_ig_lds_off<BK, true>(lq[b] + sh, 2 * kk + fh));
+  };
+  lt_ig_f32x16 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  constexpr int kWaitVm0 = (0xF << 8) | (0x7 << 4);  // vmcnt(0) (gfx9 s_waitcnt encoding)
+  constexpr int kWaitVmHV = kWaitVm0 | (HV & 15);    // vmcnt(HV)
+  auto step = [&](int kt, int B) {
+    const int kb = kt / NTAP, tap = kt - kb * NTAP;
+    const bool next_halo = kb + 1 < KB;
+    // the top of a block's second tap leaves the next block's halo in flight
+    if (tap == 1 && next_halo) __builtin_amdgcn_s_waitcnt(kWaitVmHV);
+    else __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    // tile kt's P and block kb's halo are in LDS, stage B^1 is free (see
+    // lt_igemm_tile_dl: a bare barrier keeps the next halo's loads in flight)
+    __builtin_amdgcn_s_barrier();
+    const bool more = kt + 1 < KT;
+    const int hb = kb & 1, sh = shift(kt);
+    frags(B, hb, sh, 0, 0);
+#pragma unroll
+    for (int kk = 0; kk < KK; ++kk) {
+      const int s = kk & 1;
+      if (kk == 0) {
+        if (more) load(kt + 1, B ^ 1);
+        if (tap == 0 && next_halo) hload(kb + 1, hb ^ 1);
+      }
+      if (kk + 1 < KK) frags(B, hb, sh, kk + 1, s ^ 1);
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) acc[a][b] = lt_ig_mfma(fa[s][a], fb[s][b], acc[a][b]);
+      constexpr int NM = TM * TN, NR = TM + TN;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x0008, 1, 0);  // MFMA
+        if (kk + 1 < KK && i < NR) __builtin_amdgcn_sched_group_barrier(0x0100, 1, 0);  // LDS read
+      }
+    }
+  };
+  hload(0, 0);
+  load(0, 0);
+  int kt = 0;
+#pragma unroll 1  // a K loop unrolled whole would hold every K tile's addresses
+  for (; kt + 1 < KT; kt += 2) {
+    step(kt, 0);
+    step(kt + 1, 1);
+  }
+  if (kt < KT) step(kt, 0);
   lt_igemm_store<TM, TN, FULL>(g, acc, out, i0 + wm * WM, j0 + wn * WN, lane);
 }
 

@@ -169,13 +169,15 @@ STAGED = CONTRACT
 @pytest.mark.parametrize("tile", ["", "tile=64x64", "tile=128x64", "tile=64x128", "tile=128x256",
                                   "tile=128x256 bk=64", "tile=64x64 bk=64", "igemm=plain",
                                   "tile=128x256 bk=64 igemm=plain", "contract=gather",
-                                  "stage=flat", "waves=8", "waves=2", "lds=swz",
-                                  "tile=256x256 bk=64 waves=8 lds=swz", "kb=outer"])
+                                  "stage=flat", "waves=8", "waves=2", "lds=swz", "lds=pad",
+                                  "tile=256x256 bk=64 waves=8 lds=swz", "kb=outer",
+                                  "tile=256x256 bk=32 lds=dl", "tile=64x64 bk=64 lds=dl"])
 def test_contraction_bf16_staged_tiles(lt, hip_dev, name, kw, tile):
     """Staged-operand lowering (both operands permuted -- k-blocked or flat --,
     zero-haloed and rounded to bf16 by one staging launch, then the
     implicit-GEMM tile, pipelined or single-stage, with 2, 4 or 8 waves and
-    padded or XOR-permuted LDS rows) at every tile shape, and the gathered
+    padded or XOR-permuted LDS rows filled through registers or by
+    direct-to-LDS loads) at every tile shape, and the gathered
     lowering, against float64 of the bf16-rounded operands."""
     out, shapes = getattr(P, name)(lt, **kw)
     inputs = contract_inputs(name, shapes)
@@ -218,7 +220,8 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     16-bit copy or straight from the float source) computes the same K order
     as the
     pipelined tile over the same k-blocked staging (blocks outer, taps
-    inner): the two results must be bit-identical, and within float32
+    inner), with LDS filled through registers or by direct-to-LDS loads: the
+    results must be bit-identical, and within float32
     accumulation error of float64 over the bf16-rounded operands.  Covers
     partial tiles, stride 2, a 5x5 window, rows longer than a tile, pad 0
     and channel counts that are not a multiple of the K block."""
@@ -228,7 +231,8 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     inputs = contract_inputs(name, shapes)
     ys = []
     for knobs in ("igemm=pipe stage=blocked kb=outer", "igemm=halo", "igemm=halo1",
-                  "igemm=halo halo=direct", "igemm=halo1 halo=direct"):
+                  "igemm=halo halo=direct", "igemm=halo1 halo=direct", "igemm=halo1 lds=dl",
+                  "igemm=pipe stage=blocked kb=outer lds=swz"):
         t = out.loop_tree
         t = t.annotate(t.roots[0], (knobs + " " + tile).strip())
         cc = lt.compile(t, "hip_bf16")
@@ -365,7 +369,7 @@ def _bf16_round(a):
 
 KNOB_OPTIONS = [["", "tile=64x64", "tile=128x128", "tile=64x128", "tile=128x256",
                  "tile=256x256 bk=64"],
-                ["", "waves=2", "waves=8"], ["", "lds=swz"], ["", "stage=flat"],
+                ["", "waves=2", "waves=8"], ["", "lds=swz", "lds=pad", "lds=dl"], ["", "stage=flat"],
                 ["", "kb=inner", "kb=outer"],
                 ["", "igemm=plain", "igemm=pipe", "igemm=halo", "igemm=halo1"],
                 ["", "contract=staged", "contract=gather"], ["", "streamk=1", "streamk=0"],
