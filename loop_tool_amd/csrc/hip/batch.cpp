@@ -184,16 +184,20 @@ struct lth_batch {
     }
   }
 
-  // the grouped kernel for the batch's configurations: the smallest register
+  // the grouped kernel for the batch's configurations: the one-configuration
+  // kernel when every item uses the same one, else the smallest register
   // class covering them (gemm.hip: small 0-2, mid adds 6, the full kernel
-  // all), and its dynamic LDS, sized for the largest configuration present
+  // all); its dynamic LDS is sized for the largest configuration present
   void *grouped_kernel(unsigned *lds) {
     int cls = 0;
+    bool one = !cfg.empty();
     *lds = 0;
     for (int c : cfg) {
       cls = std::max(cls, c <= 2 ? 0 : c == 6 ? 1 : 2);
+      one = one && c == cfg[0];
       *lds = std::max(*lds, (unsigned)lds_bytes(c));
     }
+    if (one) return gemm_module().fn("lt_ggemm_c" + std::to_string(cfg[0]) + "_" + tname());
     const char *kname[3] = {"lt_ggemm_small_", "lt_ggemm_mid_", "lt_ggemm_"};
     return gemm_module().fn(std::string(kname[cls]) + tname());
   }

@@ -71,6 +71,16 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
 #define LT_CFG_SMALL(_) _(0, 64, 64, 32) _(1, 128, 64, 32) _(2, 64, 128, 32)
 #define LT_CFG_MID(_) LT_CFG_SMALL(_) _(6, 128, 128, 32)
 #define LT_CFG_LIST(_) LT_CFG_MID(_) _(3, 128, 128, 64) _(4, 256, 128, 32) _(5, 128, 256, 32)
+// one configuration per kernel, for batches whose items all use it: its own
+// register count alone sets the occupancy, and the waves-per-SIMD hint asks
+// the compiler to keep it there
+#define LT_CFG_ONE0(_) _(0, 64, 64, 32)
+#define LT_CFG_ONE1(_) _(1, 128, 64, 32)
+#define LT_CFG_ONE2(_) _(2, 64, 128, 32)
+#define LT_CFG_ONE3(_) _(3, 128, 128, 64)
+#define LT_CFG_ONE4(_) _(4, 256, 128, 32)
+#define LT_CFG_ONE5(_) _(5, 128, 256, 32)
+#define LT_CFG_ONE6(_) _(6, 128, 128, 32)
 
 #define LT_PLAIN(T, TN, BM, BN, BK)                                                          \
   extern "C" __global__ void __launch_bounds__(256)                                         \
@@ -93,8 +103,8 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
 // on XCD w % 8 and takes items of that XCD's segment: first by index, then
 // from the segment's counter.  With fewer than 8 workgroups, workgroup w
 // runs segments w, w + grid, ... alone, in order.
-#define LT_GROUPED(T_, TN, CLASS, LIST)                                                      \
-  extern "C" __global__ void __launch_bounds__(256)                                         \
+#define LT_GROUPED(T_, TN, CLASS, LIST, ...)                                                 \
+  extern "C" __global__ void __launch_bounds__(256) __VA_ARGS__                             \
       lt_ggemm##CLASS##_##TN(const LtGemmDesc* __restrict__ descs,                          \
                              const LtWorkItem* __restrict__ items, int* __restrict__ queue) { \
     typedef T_ T;                                                                           \
@@ -145,12 +155,21 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
 
 LT_PLAIN_ALL(__bf16, bf16)
 LT_PLAIN_ALL(_Float16, f16)
-LT_GROUPED(__bf16, bf16, , LT_CFG_LIST)
-LT_GROUPED(__bf16, bf16, _mid, LT_CFG_MID)
-LT_GROUPED(__bf16, bf16, _small, LT_CFG_SMALL)
-LT_GROUPED(_Float16, f16, , LT_CFG_LIST)
-LT_GROUPED(_Float16, f16, _mid, LT_CFG_MID)
-LT_GROUPED(_Float16, f16, _small, LT_CFG_SMALL)
+#define LT_WPE(n) __attribute__((amdgpu_waves_per_eu(n)))
+#define LT_GROUPED_ALL(T, TN)                                                               \
+  LT_GROUPED(T, TN, , LT_CFG_LIST)                                                          \
+  LT_GROUPED(T, TN, _mid, LT_CFG_MID)                                                       \
+  LT_GROUPED(T, TN, _small, LT_CFG_SMALL)                                                   \
+  LT_GROUPED(T, TN, _c0, LT_CFG_ONE0, LT_WPE(8))                                            \
+  LT_GROUPED(T, TN, _c1, LT_CFG_ONE1, LT_WPE(5))                                            \
+  LT_GROUPED(T, TN, _c2, LT_CFG_ONE2, LT_WPE(5))                                            \
+  LT_GROUPED(T, TN, _c3, LT_CFG_ONE3, LT_WPE(3))                                            \
+  LT_GROUPED(T, TN, _c4, LT_CFG_ONE4)                                                       \
+  LT_GROUPED(T, TN, _c5, LT_CFG_ONE5)                                                       \
+  LT_GROUPED(T, TN, _c6, LT_CFG_ONE6, LT_WPE(3))
+
+LT_GROUPED_ALL(__bf16, bf16)
+LT_GROUPED_ALL(_Float16, f16)
 
 // float32 [rows][cols] -> 16-bit [rows][cols], or [cols][rows] when transpose
 #define LT_CVT(T, TN)                                                                       \
