@@ -601,7 +601,12 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   const Operand &QOP = c.ops[c.qo];
   const int64_t q_src_bytes =
       df.numel(df.nodes.at(df.ir().inputs()[QOP.input_index]).vars) * 4;
-  const bool qdirect = HL && knobs.halo_direct == 1 && q_src_bytes < (1LL << 31);
+  // (one value per tap over a block's first 8 taps: windows of 9 or more taps)
+  const int64_t qf_lds = 2 * 64 * 4 * (int64_t)(WGM * WGN) *
+                         (((HL + 63) / 64 * (BK / 8) + WGM * WGN - 1) / (WGM * WGN));
+  const bool qdirect = HL && knobs.halo_direct == 1 && q_src_bytes < (1LL << 31) &&
+                       ntap >= 9 &&
+                       (2 * BM + HB * HL) * (BK + 8) * 2 + qf_lds <= 160 * 1024;
   if (knobs.halo_direct == 1 && HL && !qdirect) return false;
   // direct-to-LDS halo tile (lds=dl): P stages and halo buffers filled by
   // loads that write LDS; the halo rounded up to whole load rounds
@@ -611,7 +616,7 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
                    2 * (BM + HLP) * BK * 2 <= 160 * 1024 &&
                    (HL * hcpr + hthreads - 1) / hthreads <= 15;
   const int lds = hdl   ? (int)(2 * (BM + HLP) * BK * 2)
-                  : HL  ? (int)((2 * BM + HB * HL) * (BK + 8) * 2)
+                  : HL  ? (int)((2 * BM + HB * HL) * (BK + 8) * 2 + (qdirect ? qf_lds : 0))
                   : f32 ? 2 * (BM + BN) * (BK + 4) * 4
                         : 2 * (BM + BN) * (swz || dl ? BK : BK + 8) * 2;
   const int threads = 64 * WGM * WGN;

@@ -401,6 +401,12 @@ __host__ __device__ constexpr int lt_igemm_dl_lds_bytes() {
 
 typedef __attribute__((address_space(3))) void lt_ig_lds_void;
 
+// s_waitcnt operand that waits for vmcnt <= n (n < 64) and leaves lgkmcnt and
+// expcnt alone (gfx9 encoding: vmcnt bits [3:0] and [15:14])
+__host__ __device__ constexpr int lt_ig_vmcnt(int n) {
+  return (n & 15) | (((n >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8);
+}
+
 template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool KTAIL = false,
           class G>
 __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict__ P,
@@ -538,11 +544,27 @@ __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict
 // Staged row r of a block (the position of one BK-vector) comes from
 // g.q_src(r, ok) -- the source offset of its first k_in, ok false on a zero
 // halo row -- and the BK values of block kb at k_in = kb * BK + kr, kr < BK,
-// G::QCS elements apart (zero past G::QKIN).  Each lane loads its 8-value
-// chunks with dword buffer loads while the block's first tap runs, and
-// rounds them to T (as the staging launch does) when it stores the halo: the
-// same operand values in the same order as the staged form.  Qf and qfbytes
-// then describe the source (under 2^31 bytes); Q and qbytes are unused.
+// G::QCS elements apart (zero past G::QKIN).  The halo is cut into wave
+// units of 64 consecutive rows by one 8-value part (unit u = wave + NW * h,
+// part u % CPR): a lane owns one row of each of its units, so every dword
+// load of a wave reads 64 consecutive source positions of one k_in.  The next
+// block's values come one k_in per step (K tile), as dword loads that write a
+// wave-private LDS landing slot (two slots, alternating): value c of block b
+// at step NTAP * (b - 1) - 1 + c, from the last tap of block b - 2 on.  Two
+// steps after its load the lane reads its value back, rounds it to T (as the
+// staging launch does) and shifts it into its 16-byte chunk, so no more than
+// the chunk itself stays in registers; the chunk is complete at block b - 1's
+// last tap, where it is stored.  The first block is loaded into registers
+// before the K loop.  The same operand
+// values in the same order as the staged form; needs NTAP >= 9.  Qf and
+// qfbytes describe the source (under 2^31 bytes); Q and qbytes are unused.
+// LDS beyond the staged form: 2 * NW * HQ * 64 floats (lt_igemm_halo_qf_lds).
+// LDS bytes of the QF landing slots of lt_igemm_tile_halo
+template <int BK, int HL, int NT>
+__host__ __device__ constexpr int lt_igemm_halo_qf_lds() {
+  return 2 * (NT / 64) * (((HL + 63) / 64 * (BK / 8) + NT / 64 - 1) / (NT / 64)) * 64 * 4;
+}
+
 template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, int HL, int HB,
           class G, bool QF = false>
 __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restrict__ P,
@@ -558,14 +580,19 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   constexpr int CPR = BK / 8, KK = BK / 16;
   constexpr int VP = BM * CPR / NT;
   constexpr int HC = HL * CPR, HV = (HC + NT - 1) / NT;  // halo 16-byte chunks
+  // QF: wave units of 64 halo rows x one part, HQ per wave
+  constexpr int NW = NT / 64, HU = (HL + 63) / 64 * CPR, HQ = (HU + NW - 1) / NW;
+  constexpr int NHV = QF ? HQ : HV;
   static_assert(TM >= 1 && TN >= 1 && WM % 32 == 0 && WN % 32 == 0, "wave tile");
   static_assert(VP >= 1 && BM * CPR % NT == 0, "tile too small for the workgroup");
   static_assert(KK % 2 == 0, "BK must be a multiple of 32");
   constexpr int NTAP = G::NTAP;
   static_assert(NTAP >= 2, "the halo form needs at least two taps per K block");
+  static_assert(!QF || NTAP >= 9, "the float-source halo loads one value per tap over 8 taps");
   const int KT = g.KO, KB = KT / NTAP;
   T* sP = (T*)smem;              // [2][BM][LDR]
   T* sH = sP + 2 * BM * LDR;     // [HB][HL][LDR]
+  float* sF = (float*)(sH + HB * HL * LDR);  // QF landing slots [2][NW][HQ][64]
   static_assert(HB == 1 || HB == 2, "one or two halo buffers");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave / WGN, wn = wave % WGN;
@@ -595,63 +622,110 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
     bool ok;
     lq[b] = (int)((g.q_row(j, ok) - qb0) / BK) * LDR + fh * 8;
   }
-  // halo chunks of this lane: chunk e = tid + h * NT is vector e / CPR, part e % CPR
-  unsigned hv[HV];
+  // halo chunks of this lane.  Staged: chunk e = tid + h * NT is vector
+  // e / CPR, part e % CPR.  QF: unit u = wave + NW * h, row (u / CPR) * 64 +
+  // lane, part u % CPR; hv is the source offset (bytes) of the row's first
+  // value of the part in block 0.
+  unsigned hv[NHV];
 #pragma unroll
-  for (int h = 0; h < HV; ++h) {
-    const int e = tid + h * NT;
+  for (int h = 0; h < NHV; ++h) {
     if constexpr (QF) {
-      // the source offset (bytes) of the chunk's first value in block 0
+      const int u = wave + NW * h, row = (u / CPR) * 64 + lane;
       bool ok = false;
-      const long src = e < HC ? g.q_src(qb0 / BK + e / CPR, ok) : 0;
-      hv[h] = ok ? (unsigned)((src + (long)((e % CPR) * 8) * G::QCS) * 4) : 0x80000000u;
+      const long src = u < HU && row < HL ? g.q_src(qb0 / BK + row, ok) : 0;
+      hv[h] = ok ? (unsigned)((src + (long)((u % CPR) * 8) * G::QCS) * 4) : 0x80000000u;
     } else {
+      const int e = tid + h * NT;
       hv[h] = e < HC ? (unsigned)((qb0 + (long)(e / CPR) * BK + (e % CPR) * 8) * 2)
                      : 0x80000000u;
     }
   }
-  lt_ig_u32x4 hr[QF ? 1 : HV];
-  float hf[QF ? HV : 1][8];
+  lt_ig_u32x4 hr[NHV];  // the next block's chunks
   const auto rsQf = __builtin_amdgcn_make_buffer_rsrc((void*)Qf, (short)0, (int)qfbytes,
                                                       0x00020000);
   // the block offset goes into the per-lane offset, which the buffer range
   // check covers: a halo tail past the end of Q (or a zero halo row of the
   // source, offset 2^31) reads zeros
+  auto qf_off = [&](int h, int kb, int v) -> unsigned {
+    if constexpr (QF) {
+      const int c = kb * BK + ((wave + NW * h) % CPR) * 8 + v;
+      return (G::QKIN % BK == 0 || c < G::QKIN)
+                 ? hv[h] + (unsigned)((long)(kb * BK + v) * G::QCS * 4)
+                 : 0x80000000u;
+    } else {
+      return 0x80000000u;
+    }
+  };
   auto hload = [&](int kb) {
     if constexpr (QF) {
+      // the whole block into registers (the first block, before the K loop)
 #pragma unroll
-      for (int h = 0; h < HV; ++h) {
-        const int c0 = kb * BK + ((tid + h * NT) % CPR) * 8;
+      for (int h = 0; h < NHV; ++h) {
+        V x;
 #pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          const unsigned off = (G::QKIN % BK == 0 || c0 + v < G::QKIN)
-                                   ? hv[h] + (unsigned)((long)(kb * BK + v) * G::QCS * 4)
-                                   : 0x80000000u;
-          hf[h][v] = __builtin_bit_cast(
-              float, __builtin_amdgcn_raw_buffer_load_b32(rsQf, (int)off, 0, 0));
-        }
+        for (int v = 0; v < 8; ++v)
+          x[v] = (T)__builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(rsQf, (int)qf_off(h, kb, v), 0, 0));
+        hr[h] = __builtin_bit_cast(lt_ig_u32x4, x);
       }
     } else {
       const unsigned bo = (unsigned)(g.q_koff(kb * NTAP) * 2);
 #pragma unroll
-      for (int h = 0; h < HV; ++h)
+      for (int h = 0; h < NHV; ++h)
         hr[h] = __builtin_amdgcn_raw_buffer_load_b128(rsQ, (int)(hv[h] + bo), 0, 0);
     }
   };
+  // QF: value v of every unit of block kb into landing slot v & 1 (each
+  // lane's dword lands at the slot's base + 4 * lane)
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);  // the slot base is wave-uniform
+  auto hload_tap = [&](int kb, int v, int slot) {
+#pragma unroll
+    for (int h = 0; h < NHV; ++h)
+      if (HU % NW == 0 || wave_s + NW * h < HU)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rsQf, (lt_ig_lds_void*)(sF + ((slot * NW + wave_s) * NHV + h) * 64), 4,
+            (int)qf_off(h, kb, v), 0, 0, 0);
+  };
+  // QF: the value (0..7) of the next blocks' stream loaded at step kt, -1
+  // when none is: step kt loads value (kt + 1) % NTAP of block
+  // (kt + 1) / NTAP + 1, when that value is below 8 and the block exists
+  auto qf_val = [&](int kt) {
+    const int c = (kt + 1) % NTAP, b = (kt + 1) / NTAP + 1;
+    return kt >= -1 && c < 8 && b < KB ? c : -1;
+  };
+  // QF: the value of slot (v & 1), rounded to T, shifted in at the top of the
+  // chunk (after the 8 values 0..7 of a block, value v is element v)
+  auto hconv = [&](int slot) {
+    float x[NHV];
+#pragma unroll
+    for (int h = 0; h < NHV; ++h)  // all reads first, one wait
+      x[h] = HU % NW == 0 || wave + NW * h < HU
+                 ? sF[((slot * NW + wave) * NHV + h) * 64 + lane]
+                 : 0.0f;
+#pragma unroll
+    for (int h = 0; h < NHV; ++h) {
+      const unsigned b = (unsigned)__builtin_bit_cast(unsigned short, (T)x[h]);
+      hr[h][0] = (hr[h][0] >> 16) | (hr[h][1] << 16);
+      hr[h][1] = (hr[h][1] >> 16) | (hr[h][2] << 16);
+      hr[h][2] = (hr[h][2] >> 16) | (hr[h][3] << 16);
+      hr[h][3] = (hr[h][3] >> 16) | (b << 16);
+    }
+    // the chunks as registers here: the compiler may not keep the 8 float
+    // values of a chunk live instead and convert them at the store
+#pragma unroll
+    for (int h = 0; h < NHV; ++h) asm volatile("" : "+v"(hr[h])::"memory");
+  };
   auto hstore = [&](int buf) {
 #pragma unroll
-    for (int h = 0; h < HV; ++h) {
-      const int e = tid + h * NT;
-      if (HC % NT == 0 || e < HC) {
-        lt_ig_u32x4* dst = (lt_ig_u32x4*)(sH + buf * HL * LDR + (e / CPR) * LDR + (e % CPR) * 8);
-        if constexpr (QF) {
-          V x;
-#pragma unroll
-          for (int v = 0; v < 8; ++v) x[v] = (T)hf[h][v];
-          *dst = __builtin_bit_cast(lt_ig_u32x4, x);
-        } else {
-          *dst = hr[h];
-        }
+    for (int h = 0; h < NHV; ++h) {
+      if constexpr (QF) {
+        const int u = wave + NW * h, row = (u / CPR) * 64 + lane;
+        if ((HU % NW == 0 || u < HU) && row < HL)
+          *(lt_ig_u32x4*)(sH + buf * HL * LDR + row * LDR + (u % CPR) * 8) = hr[h];
+      } else {
+        const int e = tid + h * NT;
+        if (HC % NT == 0 || e < HC)
+          *(lt_ig_u32x4*)(sH + buf * HL * LDR + (e / CPR) * LDR + (e % CPR) * 8) = hr[h];
       }
     }
   };
@@ -686,6 +760,20 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
+  // The workgroup barrier between K tiles.  QF: this wave's LDS accesses
+  // complete (lgkmcnt 0), then a bare s_barrier.  __syncthreads would also
+  // wait for every load in flight (vmcnt 0), the direct-to-LDS halo loads
+  // included, which land in wave-private slots that no other wave reads.
+  auto bar = [&]() {
+    if constexpr (QF) {
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    } else {
+      __syncthreads();
+    }
+  };
   // One K tile kt: its P tile is in LDS buffer B, its block's halo in halo
   // buffer (kt / NTAP) % 2, its k16-step-0 fragments in fa/fb[0]; RN holds P
   // of tile kt+1, RF receives P of tile kt+2 when ld.  With st, tile kt+1's P
@@ -693,9 +781,29 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   auto step = [&](int kt, int B, const lt_ig_u32x4* RN, lt_ig_u32x4* RF, bool ld, bool st) {
     const int kb = kt / NTAP, tap = kt - kb * NTAP;
     const int ho = hoff(kt);
-    if (ld) load(kt + 2, RF);
     const bool next_halo = kb + 1 < KB;
-    if (tap == 0 && next_halo) hload(kb + 1);
+    if constexpr (QF) {
+      // the float halo stream: value c of block b's chunks is loaded at step
+      // NTAP * (b - 1) - 1 + c (c < 8: from the last tap of block b - 2 on)
+      // into slot (step & 1), and read back two steps later
+      if (kt >= 1 && qf_val(kt - 2) >= 0) {
+        // wait until this wave's halo loads of step kt - 2 have landed: the
+        // loads issued after them (step kt - 1: P, then halo) may stay out
+        const bool h1 = qf_val(kt - 1) >= 0, p1 = kt + 1 < KT;
+        if (h1 && p1) __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(NHV + VP));
+        else if (h1) __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(NHV));
+        else if (p1) __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(VP));
+        else __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(0));
+        asm volatile("" ::: "memory");
+        hconv(kt & 1);
+      }
+      if (ld) load(kt + 2, RF);
+      // (the slot was read back above: hconv's results are pinned first)
+      if (qf_val(kt) >= 0) hload_tap((kt + 1) / NTAP + 1, qf_val(kt), kt & 1);
+    } else {
+      if (ld) load(kt + 2, RF);
+      if (tap == 0 && next_halo) hload(kb + 1);
+    }
     // raised wave priority while this wave issues its MFMAs: the sequencer
     // favours it over waves that are loading or storing (measured 3-4 %
     // faster on the 3x3 conv tile)
@@ -713,11 +821,11 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
     __builtin_amdgcn_s_setprio(0);
     if (HB == 2 && tap == NTAP - 1 && next_halo) hstore((kb + 1) & 1);
     if (st) {
-      __syncthreads();
+      bar();
       if (HB == 1 && tap == NTAP - 1 && next_halo) {
         // one halo buffer: every wave is past block kb's last reads
         hstore(0);
-        __syncthreads();
+        bar();
       }
       frags(B ^ 1, hoff(kt + 1), 0, 0);
     }
@@ -725,9 +833,11 @@ __device__ __forceinline__ void lt_igemm_tile_halo(const G& g, const T* __restri
   hload(0);
   load(0, r0p);
   if (KT > 1) load(1, r1p);
+  if constexpr (QF)
+    if (qf_val(-1) >= 0) hload_tap(1, 0, 1);  // the stream's first value (step -1)
   hstore(0);
   store(0, r0p);
-  __syncthreads();
+  bar();
   frags(0, hoff(0), 0, 0);
   // tile t's P lives in register set t % 2 and LDS buffer t % 2
   int kt = 0;

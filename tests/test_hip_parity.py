@@ -233,6 +233,11 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     for knobs in ("igemm=pipe stage=blocked kb=outer", "igemm=halo", "igemm=halo1",
                   "igemm=halo halo=direct", "igemm=halo1 halo=direct", "igemm=halo1 lds=dl",
                   "igemm=pipe stage=blocked kb=outer lds=swz"):
+        if (knobs == "igemm=halo halo=direct" and tile == "tile=128x256 waves=4"
+                and kw.get("W") == 300):
+            # two halo buffers of 864 rows plus the float source's landing
+            # slots exceed the LDS of a CU (the one-buffer form runs)
+            continue
         t = out.loop_tree
         t = t.annotate(t.roots[0], (knobs + " " + tile).strip())
         cc = lt.compile(t, "hip_bf16")
