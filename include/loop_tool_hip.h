@@ -23,7 +23,9 @@ extern "C" {
 
 /* 1 if the Compiled was produced by the hip backend */
 int lth_is_hip(const ltc_compiled *cc);
-/* the Compiled's own stream (created on first use) */
+/* the calling thread's stream of the Compiled (created on the thread's first
+ * use): ltc_compiled_run launches there, with that thread's own scratch, so
+ * runs from different threads proceed concurrently */
 int lth_compiled_stream(const ltc_compiled *cc, void **stream);
 /* number of plan stages (kernel launches per run) */
 int lth_num_stages(const ltc_compiled *cc);

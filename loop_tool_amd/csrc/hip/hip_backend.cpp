@@ -391,13 +391,20 @@ bool runtime_exiting() { return g_exiting.load(); }
 
 HipCompiled::~HipCompiled() {
   if (!loaded_ || runtime_exiting()) return;
-  for (auto p : scratch_) lt_hip_free(p);
-  for (auto p : scratch2_) lt_hip_free(p);
-  for (auto &p : staging_) lt_hip_free(p.second.first);
-  for (auto *e : {ev_staged_[0], ev_staged_[1], ev_done_[0], ev_done_[1]})
-    if (e) lt_hip_event_destroy(e);
-  if (side_) lt_hip_stream_destroy(side_);
-  if (stream_) lt_hip_stream_destroy(stream_);
+  for (auto &kv : contexts_) {
+    auto &c = *kv.second;
+    // work still queued on the context's streams (a run with sync=false)
+    // finishes before its buffers go
+    if (c.side) lt_hip_stream_synchronize(c.side);
+    if (c.stream) lt_hip_stream_synchronize(c.stream);
+    for (auto p : c.scratch) lt_hip_free(p);
+    for (auto p : c.scratch2) lt_hip_free(p);
+    for (auto &p : c.staging) lt_hip_free(p.second.first);
+    for (auto *e : {c.ev_staged[0], c.ev_staged[1], c.ev_done[0], c.ev_done[1]})
+      if (e) lt_hip_event_destroy(e);
+    if (c.side) lt_hip_stream_destroy(c.side);
+    if (c.stream) lt_hip_stream_destroy(c.stream);
+  }
   if (module_) lt_hip_module_unload(module_);
 }
 
@@ -418,15 +425,26 @@ void HipCompiled::load() const {
     hip_check(lt_hip_module_get_function(module_, s.kernel.c_str(), &fn), "get function");
     functions_.push_back(fn);
   }
-  hip_check(lt_hip_stream_create(&stream_), "stream");
   loaded_ = true;
+}
+
+HipCompiled::RunContext &HipCompiled::context() const {
+  std::lock_guard<std::mutex> g(mu_);
+  load();
+  auto &slot = contexts_[std::this_thread::get_id()];
+  if (!slot) {
+    auto c = std::make_unique<RunContext>();
+    hip_check(lt_hip_stream_create(&c->stream), "stream");
+    slot = std::move(c);
+  }
+  return *slot;
 }
 
 // The Compiled's own scratch, allocated on first use by run()/run_stage():
 // run_on() callers (the serving runtime) bring theirs, so a program compiled
 // only for them holds no device memory beyond its code.
-void HipCompiled::ensure_scratch(int set) const {
-  auto &sc = set ? scratch2_ : scratch_;
+void HipCompiled::ensure_scratch(RunContext &c, int set) const {
+  auto &sc = set ? c.scratch2 : c.scratch;
   if (!sc.empty()) return;
   for (const auto &b : plan_.buffers) {
     if (b.kind != BufferSpec::SCRATCH) continue;
@@ -453,9 +471,10 @@ void HipCompiled::launch(const std::vector<void *> &device_buffers, void *stream
   }
 }
 
-std::vector<void *> HipCompiled::device_buffers(const std::vector<void *> &io, int set) const {
-  ensure_scratch(set);
-  const auto &sc = set ? scratch2_ : scratch_;
+std::vector<void *> HipCompiled::device_buffers(RunContext &c, const std::vector<void *> &io,
+                                                int set) const {
+  ensure_scratch(c, set);
+  const auto &sc = set ? c.scratch2 : c.scratch;
   std::vector<void *> bufs;
   size_t scratch_i = 0;
   for (const auto &b : plan_.buffers) {
@@ -481,9 +500,12 @@ size_t HipCompiled::num_io() const {
   return n;
 }
 
+// Every call runs on the calling thread's context (stream, scratch, staging
+// copies): calls from one thread are ordered on its stream, calls from
+// different threads run concurrently.  sync=false returns once the work is
+// queued on the thread's stream.
 void HipCompiled::run(const std::vector<void *> &memory, bool sync) const {
-  std::lock_guard<std::mutex> g(mu_);
-  load();
+  RunContext &c = context();
   ASSERT(memory.size() >= num_io())
       << "hip backend expects " << num_io() << " buffers (inputs then outputs), got "
       << memory.size();
@@ -494,14 +516,18 @@ void HipCompiled::run(const std::vector<void *> &memory, bool sync) const {
     if (!io[i] || lt_hip_is_device_pointer(io[i])) continue;
     const auto &spec = plan_.buffers[i];
     size_t bytes = spec.numel * sizeof(float);
-    auto &st = staging_[i];
+    auto &st = c.staging[i];
     if (st.second < bytes) {
-      if (st.first) lt_hip_free(st.first);
+      // the stream may still read the old copy (an earlier sync=false run)
+      if (st.first) {
+        hip_check(lt_hip_stream_synchronize(c.stream), "synchronize");
+        lt_hip_free(st.first);
+      }
       hip_check(lt_hip_malloc(&st.first, bytes), "staging allocation");
       st.second = bytes;
     }
     if (spec.kind == BufferSpec::INPUT)
-      hip_check(lt_hip_memcpy_async(st.first, io[i], bytes, 0, stream_), "copy in");
+      hip_check(lt_hip_memcpy_async(st.first, io[i], bytes, 0, c.stream), "copy in");
     else
       copy_out.emplace_back(i, io[i]);
     io[i] = st.first;
@@ -510,34 +536,34 @@ void HipCompiled::run(const std::vector<void *> &memory, bool sync) const {
   if (overlap_ && !staged) {
     // Runs alternate between two scratch sets.  The first stage (operand
     // staging) runs on the side stream as soon as the run that last used
-    // its set has finished; the later stages run on the Compiled's stream
+    // its set has finished; the later stages run on the context's stream
     // after it.  Run i+1's staging thus overlaps run i's main kernels.
-    if (!side_) {
-      hip_check(lt_hip_stream_create(&side_), "stream");
+    if (!c.side) {
+      hip_check(lt_hip_stream_create(&c.side), "stream");
       for (int b = 0; b < 2; ++b) {
-        hip_check(lt_hip_event_create(&ev_staged_[b]), "event");
-        hip_check(lt_hip_event_create(&ev_done_[b]), "event");
-        hip_check(lt_hip_event_record(ev_done_[b], stream_), "event record");
+        hip_check(lt_hip_event_create(&c.ev_staged[b]), "event");
+        hip_check(lt_hip_event_create(&c.ev_done[b]), "event");
+        hip_check(lt_hip_event_record(c.ev_done[b], c.stream), "event record");
       }
     }
-    const int set = (int)(runs_++ & 1);
-    auto bufs = device_buffers(io, set);
-    hip_check(lt_hip_stream_wait_event(side_, ev_done_[set]), "stream wait");
-    launch(bufs, side_, 0, 0);
-    hip_check(lt_hip_event_record(ev_staged_[set], side_), "event record");
-    hip_check(lt_hip_stream_wait_event(stream_, ev_staged_[set]), "stream wait");
-    launch(bufs, stream_, 1);
-    hip_check(lt_hip_event_record(ev_done_[set], stream_), "event record");
+    const int set = (int)(c.runs++ & 1);
+    auto bufs = device_buffers(c, io, set);
+    hip_check(lt_hip_stream_wait_event(c.side, c.ev_done[set]), "stream wait");
+    launch(bufs, c.side, 0, 0);
+    hip_check(lt_hip_event_record(c.ev_staged[set], c.side), "event record");
+    hip_check(lt_hip_stream_wait_event(c.stream, c.ev_staged[set]), "stream wait");
+    launch(bufs, c.stream, 1);
+    hip_check(lt_hip_event_record(c.ev_done[set], c.stream), "event record");
   } else {
-    launch(device_buffers(io), stream_);
+    launch(device_buffers(c, io), c.stream);
     // a later run-ahead staging into set 0 must wait for this run
-    if (side_) hip_check(lt_hip_event_record(ev_done_[0], stream_), "event record");
+    if (c.side) hip_check(lt_hip_event_record(c.ev_done[0], c.stream), "event record");
   }
   for (const auto &p : copy_out)
     hip_check(lt_hip_memcpy_async(p.second, io[p.first],
-                                  plan_.buffers[p.first].numel * sizeof(float), 1, stream_),
+                                  plan_.buffers[p.first].numel * sizeof(float), 1, c.stream),
               "copy out");
-  if (sync || staged) hip_check(lt_hip_stream_synchronize(stream_), "synchronize");
+  if (sync || staged) hip_check(lt_hip_stream_synchronize(c.stream), "synchronize");
 }
 
 void HipCompiled::run_on(const std::vector<void *> &device_io,
@@ -561,12 +587,11 @@ void HipCompiled::run_on(const std::vector<void *> &device_io,
 }
 
 void HipCompiled::run_stage(const std::vector<void *> &device_io, int k, void *stream) const {
-  std::lock_guard<std::mutex> g(mu_);
-  load();
+  RunContext &c = context();
   ASSERT(k >= 0 && k < (int)plan_.stages.size()) << "no stage " << k;
   ASSERT(device_io.size() >= num_io()) << "expected " << num_io() << " device buffers";
-  launch(device_buffers(device_io), stream, k, k);
-  if (side_) hip_check(lt_hip_event_record(ev_done_[0], stream), "event record");
+  launch(device_buffers(c, device_io), stream, k, k);
+  if (c.side) hip_check(lt_hip_event_record(c.ev_done[0], stream), "event record");
 }
 
 std::vector<int64_t> HipCompiled::scratch_sizes() const {

@@ -3,6 +3,7 @@
 
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -39,7 +40,8 @@ class HipCompiled : public Compiled {
               void *stream) const;
   std::vector<int64_t> scratch_sizes() const;
   // Launch only stage k of the plan on `stream` with device buffers (inputs
-  // then outputs) and the Compiled's own scratch (timing of one kernel).
+  // then outputs) and the calling thread's scratch of the Compiled (timing of
+  // one kernel).
   void run_stage(const std::vector<void *> &device_io, int k, void *stream) const;
 
   // true once the plan's code object is loaded on the device (by any run,
@@ -54,19 +56,33 @@ class HipCompiled : public Compiled {
   std::string schedule_note;
   int64_t algorithmic_bytes() const { return algorithmic_bytes_; }
   int64_t algorithmic_flops() const { return algorithmic_flops_; }
-  void *stream() const {
-    load();
-    return stream_;
-  }
+  // the calling thread's stream of this Compiled (run() launches on it)
+  void *stream() const { return context().stream; }
   size_t num_inputs() const;
   size_t num_io() const;
 
  private:
-  void load() const;
-  void ensure_scratch(int set = 0) const;
+  // What run() uses on one calling thread: a stream, the plan's scratch
+  // (two sets with overlap), device copies of host buffers and the overlap
+  // stream and events.  Created on the thread's first call and used by that
+  // thread only, so runs from different threads proceed concurrently, each
+  // in order on its own stream.
+  struct RunContext {
+    void *stream = nullptr;
+    std::vector<void *> scratch, scratch2;
+    std::unordered_map<size_t, std::pair<void *, size_t>> staging;
+    void *side = nullptr;
+    void *ev_staged[2] = {nullptr, nullptr};
+    void *ev_done[2] = {nullptr, nullptr};
+    uint64_t runs = 0;
+  };
+  RunContext &context() const;  // the calling thread's (loads the code object first)
+  void load() const;            // with mu_ held
+  void ensure_scratch(RunContext &c, int set = 0) const;
   void launch(const std::vector<void *> &device_buffers, void *stream, int first = 0,
               int last = -1) const;
-  std::vector<void *> device_buffers(const std::vector<void *> &io, int set = 0) const;
+  std::vector<void *> device_buffers(RunContext &c, const std::vector<void *> &io,
+                                     int set = 0) const;
 
   Plan plan_;
   Knobs knobs_;
@@ -74,20 +90,13 @@ class HipCompiled : public Compiled {
   int64_t algorithmic_bytes_ = 0;
   int64_t algorithmic_flops_ = 0;
 
-  mutable std::mutex mu_;
+  mutable std::mutex mu_;  // guards loading and contexts_
   mutable bool loaded_ = false;
   mutable lt_hip_module module_ = nullptr;
   mutable std::vector<void *> functions_;
-  mutable std::vector<void *> scratch_;
   // overlap of the staging stage with the previous run (see run())
   bool overlap_ = false;
-  mutable std::vector<void *> scratch2_;
-  mutable void *side_ = nullptr;
-  mutable void *ev_staged_[2] = {nullptr, nullptr};
-  mutable void *ev_done_[2] = {nullptr, nullptr};
-  mutable uint64_t runs_ = 0;
-  mutable void *stream_ = nullptr;
-  mutable std::unordered_map<size_t, std::pair<void *, size_t>> staging_;
+  mutable std::unordered_map<std::thread::id, std::unique_ptr<RunContext>> contexts_;
 };
 
 }  // namespace hip

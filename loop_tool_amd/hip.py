@@ -83,6 +83,11 @@ def synchronize():
     _check(_L.lt_hip_device_synchronize())
 
 
+def stream_synchronize(stream):
+    """Wait for the work queued on a raw stream handle."""
+    _check(_L.lt_hip_stream_synchronize(stream))
+
+
 def mem_info():
     f, t = C.c_size_t(), C.c_size_t()
     _check(_L.lt_hip_mem_info(C.byref(f), C.byref(t)))
@@ -212,6 +217,7 @@ def stages(compiled):
 
 
 def compiled_stream(compiled):
+    """The calling thread's stream of a hip Compiled (its runs launch there)."""
     s = _P()
     _lth(_L.lth_compiled_stream(compiled.h, C.byref(s)))
     return s.value
@@ -235,7 +241,7 @@ def time_ms(compiled, io, warmup=3, iters=20):
 
 def time_stage_ms(compiled, io, stage, warmup=3, iters=20):
     """Mean ms of one launch of plan stage `stage` alone (HIP events on the
-    Compiled's stream)."""
+    calling thread's stream of the Compiled)."""
     ms = C.c_float()
     _lth(_L.lth_time_stage(compiled.h, _ptrs(io), len(io), stage, warmup, iters,
                            C.byref(ms)))

@@ -459,6 +459,59 @@ def test_run_without_sync_on_device_buffers(lt, hip_dev):
     assert np.array_equal(yh.view(np.uint32), y.view(np.uint32))
 
 
+def test_runs_from_threads_run_apart(lt, hip_dev):
+    """Compiled::run called from several host threads at once: each thread
+    launches on its own stream with its own scratch and device copies of
+    host buffers, so one Compiled serves them concurrently (the reference's
+    cpp backend is not re-entrant, cpp.cpp:454-455).  Every thread's results
+    equal the single-threaded run bit for bit, for host buffers (sync) and
+    for device buffers queued with sync=false and then synchronised."""
+    import threading
+    out, shapes = P.conv2d(lt, 4, 64, 64, 28, 28)
+    fan = 64 * 9
+    sets = [[P.normal(60 + 2 * i, shapes[0], fan ** -0.5),
+             P.normal(61 + 2 * i, shapes[1], fan ** -0.5)] for i in range(4)]
+    t = out.loop_tree
+    cc = lt.compile(t, "hip_bf16")
+    n_out = t.io_numel()[2]
+    want = []
+    for x, w in sets:
+        y = np.zeros(n_out, np.float32)
+        cc.run([x, w, y])
+        want.append(y)
+    d_in = [[hip_dev.DeviceArray.from_numpy(a) for a in s] for s in sets]
+    d_out = [hip_dev.DeviceArray(n_out) for _ in sets]
+    got, streams, errors = {}, {}, []
+    start = threading.Barrier(len(sets))
+
+    def worker(i):
+        try:
+            streams[i] = hip_dev.compiled_stream(cc)
+            start.wait()
+            for r in range(3):
+                y = np.zeros(n_out, np.float32)
+                cc.run(sets[i] + [y])
+                got[(i, r)] = y
+            for _ in range(3):
+                cc.run([d_in[i][0].ptr, d_in[i][1].ptr, d_out[i].ptr], sync=False)
+            hip_dev.stream_synchronize(streams[i])
+            got[(i, "dev")] = d_out[i].download()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(sets))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    assert len(set(streams.values())) == len(sets)  # one stream per thread
+    assert hip_dev.compiled_stream(cc) not in streams.values()
+    for i in range(len(sets)):
+        for r in list(range(3)) + ["dev"]:
+            assert np.array_equal(got[(i, r)].view(np.uint32), want[i].view(np.uint32)), (i, r)
+
+
 def test_overlapped_runs_keep_each_run_apart(lt, hip_dev):
     """With overlap=1 a staged bf16 plan overlaps a run's operand staging
     with the previous run's MFMA kernel (two scratch sets, a second stream).
