@@ -20,8 +20,10 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <mutex>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -38,8 +40,14 @@ struct Desc {
   float *C;
   int M, N, K, pad;
 };
+// a work item as the grouped kernels read it (LtWorkItem of gemm.hip): the
+// GEMM's operands and sizes, the tile's first row and column, its tile
+// configuration
 struct Item {
-  int gemm, m0, n0, cfg;
+  const void *A;
+  const void *Bt;
+  float *C;
+  int M, N, K, m0, n0, cfg;
 };
 
 // tile configurations of csrc/kernels/gemm.hip (BM x BN x BK)
@@ -48,6 +56,11 @@ constexpr int kXcd = 8;  // XCDs of an MI355X (workgroup w runs on XCD w % 8)
 const int kTileM[kNumCfg] = {64, 128, 64, 128, 256, 128, 128};
 const int kTileN[kNumCfg] = {64, 64, 128, 128, 128, 256, 128};
 const int kTileK[kNumCfg] = {32, 32, 32, 64, 32, 32, 32};
+// The fixed cost of a work item in K tiles, for balancing the per-workgroup
+// lists: uniform batches of 128x128x64 items measured 8.4 us per item with
+// one K tile and 22.7 us with eight, i.e. about 2 us per K tile plus 6.4 us
+// (three K tiles) per item.
+constexpr double kItemFixed = 3.0;
 
 // LDS of the direct-to-LDS tile of configuration c (igemm_tile.h,
 // lt_igemm_dl_lds_bytes: two stages of both operand tiles, unpadded rows)
@@ -97,9 +110,16 @@ struct lth_batch {
   int dtype = 0;  // 0 bf16, 1 f16
   std::vector<Desc> descs;
   std::vector<int> cfg;
-  void *d_descs = nullptr, *d_items = nullptr;
+  void *d_items = nullptr;
   void *d_queue = nullptr;  // grouped kernel's segment offsets and queues (see upload)
   int n_items = 0;
+  // the work list in segment order on the host, and its segment offsets
+  std::vector<Item> h_items;
+  std::vector<int> h_xoff;
+  // per-workgroup lists of the one-configuration kernels, for lists_grid
+  // workgroups (0: none built for the current work list)
+  void *d_sitems = nullptr, *d_lists = nullptr;
+  int lists_grid = 0;
   bool dirty = true;
   void *graph = nullptr;
   void *graph_stream = nullptr;
@@ -148,8 +168,9 @@ struct lth_batch {
       prev_g = st.g;
       prev_x = x;
       const int c = cfg[st.g], bm = kTileM[c], bn = kTileN[c];
-      for (int n0 = 0; n0 < descs[st.g].N; n0 += bn) {
-        seg[x].push_back({st.g, st.m0, n0, c});
+      const Desc &d = descs[st.g];
+      for (int n0 = 0; n0 < d.N; n0 += bn) {
+        seg[x].push_back({d.A, d.Bt, d.C, d.M, d.N, d.K, st.m0, n0, c});
         segcost[x].push_back((int64_t)bm * bn * descs[st.g].K);
       }
     }
@@ -163,11 +184,8 @@ struct lth_batch {
       for (auto i : order) sorted.push_back(seg[x][i]);
       xoff[x + 1] = (int)sorted.size();
     }
-    if (d_descs) lt_hip_free(d_descs);
     if (d_items) lt_hip_free(d_items);
-    chk(lt_hip_malloc(&d_descs, descs.size() * sizeof(Desc)), "descs");
     chk(lt_hip_malloc(&d_items, std::max<size_t>(sorted.size(), 1) * sizeof(Item)), "items");
-    chk(lt_hip_memcpy(d_descs, descs.data(), descs.size() * sizeof(Desc), 0), "descs copy");
     if (!sorted.empty())
       chk(lt_hip_memcpy(d_items, sorted.data(), sorted.size() * sizeof(Item), 0), "items copy");
     // queue: [kXcd + 1] segment offsets, then kXcd segment counters and the
@@ -177,6 +195,9 @@ struct lth_batch {
     std::copy(xoff.begin(), xoff.end(), q.begin());
     chk(lt_hip_memcpy(d_queue, q.data(), q.size() * sizeof(int), 0), "queue reset");
     n_items = (int)sorted.size();
+    h_items = sorted;
+    h_xoff = xoff;
+    lists_grid = 0;
     dirty = false;
     if (graph) {
       lt_hip_graph_destroy(graph);
@@ -188,40 +209,92 @@ struct lth_batch {
   // kernel when every item uses the same one, else the smallest register
   // class covering them (gemm.hip: small 0-2, mid adds 6, the full kernel
   // all); its dynamic LDS is sized for the largest configuration present
+  bool one_config() const {
+    for (int c : cfg)
+      if (c != cfg[0]) return false;
+    return !cfg.empty();
+  }
+
   void *grouped_kernel(unsigned *lds) {
     int cls = 0;
-    bool one = !cfg.empty();
     *lds = 0;
     for (int c : cfg) {
       cls = std::max(cls, c <= 2 ? 0 : c == 6 ? 1 : 2);
-      one = one && c == cfg[0];
       *lds = std::max(*lds, (unsigned)lds_bytes(c));
     }
-    if (one) return gemm_module().fn("lt_ggemm_c" + std::to_string(cfg[0]) + "_" + tname());
+    if (one_config())
+      return gemm_module().fn("lt_ggemm_c" + std::to_string(cfg[0]) + "_" + tname());
     const char *kname[3] = {"lt_ggemm_small_", "lt_ggemm_mid_", "lt_ggemm_"};
     return gemm_module().fn(std::string(kname[cls]) + tname());
   }
 
-  // grid <= 0: as many workgroups as the GPU holds at once (workgroups
-  // beyond that would find the queue empty or nearly so)
-  int resolve_grid(int grid) {
+  // The per-workgroup lists of the one-configuration kernels for a grid of
+  // G workgroups.  The items of XCD segment x go to the workgroups on that
+  // XCD (w % 8 == x; with fewer than 8 workgroups, workgroup x % G takes the
+  // whole segment), longest first, each to the least-loaded of them at a
+  // cost of kItemFixed plus its K tiles; a list keeps the segment's order
+  // (decreasing cost).  Uploaded with blocking copies: built before a graph
+  // capture, never during one.
+  void static_lists(int G) {
+    if (lists_grid == G) return;
+    std::vector<std::vector<Item>> per(G);
+    typedef std::pair<double, int> Load;
+    for (int x = 0; x < kXcd; ++x) {
+      std::priority_queue<Load, std::vector<Load>, std::greater<Load>> least;
+      if (G >= kXcd)
+        for (int w = x; w < G; w += kXcd) least.push({0.0, w});
+      else
+        least.push({0.0, x % G});
+      for (int i = h_xoff[x]; i < h_xoff[x + 1]; ++i) {
+        const Item &it = h_items[i];
+        const Load l = least.top();
+        least.pop();
+        per[l.second].push_back(it);
+        const int kt = (it.K + kTileK[it.cfg] - 1) / kTileK[it.cfg];
+        least.push({l.first + kItemFixed + kt, l.second});
+      }
+    }
+    std::vector<Item> flat;
+    std::vector<int> off(G + 1, 0);
+    for (int w = 0; w < G; ++w) {
+      flat.insert(flat.end(), per[w].begin(), per[w].end());
+      off[w + 1] = (int)flat.size();
+    }
+    if (d_sitems) lt_hip_free(d_sitems);
+    if (d_lists) lt_hip_free(d_lists);
+    chk(lt_hip_malloc(&d_sitems, std::max<size_t>(flat.size(), 1) * sizeof(Item)), "lists");
+    chk(lt_hip_malloc(&d_lists, off.size() * sizeof(int)), "list offsets");
+    if (!flat.empty())
+      chk(lt_hip_memcpy(d_sitems, flat.data(), flat.size() * sizeof(Item), 0), "lists copy");
+    chk(lt_hip_memcpy(d_lists, off.data(), off.size() * sizeof(int), 0), "offsets copy");
+    lists_grid = G;
+  }
+
+  // the workgroups of a grouped launch: grid <= 0 means as many as the GPU
+  // holds at once (workgroups beyond that would find the queue empty or
+  // nearly so), and never more than there are items; prepares everything
+  // the launch reads (work list, per-workgroup lists)
+  int launch_grid(int grid) {
     upload();
-    if (grid > 0) return grid;
-    unsigned lds = 0;
-    void *fn = grouped_kernel(&lds);
-    chk(lt_hip_max_resident_blocks(fn, 256, lds, &grid), "occupancy");
+    if (grid <= 0) {
+      unsigned lds = 0;
+      void *fn = grouped_kernel(&lds);
+      chk(lt_hip_max_resident_blocks(fn, 256, lds, &grid), "occupancy");
+    }
+    grid = std::max(1, std::min(grid, std::max(n_items, 1)));
+    if (n_items && one_config()) static_lists(grid);
     return grid;
   }
 
   void grouped(int grid, void *stream) {
-    grid = resolve_grid(grid);
+    grid = launch_grid(grid);
     if (!n_items) return;
     unsigned lds = 0;
     void *fn = grouped_kernel(&lds);
-    int g = std::max(1, std::min(grid, n_items));
-    void *a0 = d_descs, *a1 = d_items, *a2 = d_queue;
-    void *args[] = {&a0, &a1, &a2};
-    chk(lt_hip_launch(fn, g, 1, 1, 256, 1, 1, lds, stream, args), "grouped launch");
+    const bool one = one_config();
+    void *a0 = one ? d_sitems : d_items, *a1 = one ? d_lists : d_queue;
+    void *args[] = {&a0, &a1};
+    chk(lt_hip_launch(fn, grid, 1, 1, 256, 1, 1, lds, stream, args), "grouped launch");
   }
 
   void plain(int cfg_all, void *stream) {
@@ -243,9 +316,10 @@ struct lth_batch {
   ~lth_batch() {
     if (loop_tool::hip::runtime_exiting()) return;
     if (graph) lt_hip_graph_destroy(graph);
-    if (d_descs) lt_hip_free(d_descs);
     if (d_items) lt_hip_free(d_items);
     if (d_queue) lt_hip_free(d_queue);
+    if (d_sitems) lt_hip_free(d_sitems);
+    if (d_lists) lt_hip_free(d_lists);
   }
 };
 
@@ -338,8 +412,9 @@ int lth_batch_run_grouped(lth_batch *b, int grid, void *stream) {
 int lth_batch_capture(lth_batch *b, int grid, void *stream) {
   return guarded([&] {
     ASSERT(stream) << "graph capture needs a created stream (not the null stream)";
-    // uploads, module load and the occupancy query happen outside capture
-    grid = b->resolve_grid(grid);
+    // uploads, module load, the occupancy query and the per-workgroup lists
+    // happen outside capture
+    grid = b->launch_grid(grid);
     if (b->graph) {
       lt_hip_graph_destroy(b->graph);
       b->graph = nullptr;

@@ -483,10 +483,8 @@ __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict
   for (int a = 0; a < TM; ++a)
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
-  // vmcnt(0) with lgkmcnt and expcnt left alone (gfx9 s_waitcnt encoding)
-  constexpr int kWaitVm0 = (0xF << 8) | (0x7 << 4);
   auto step = [&](int kt, int B) {
-    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's loads of tile kt have landed
+    __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(0));  // this wave's loads of tile kt have landed
     // everyone's have, and every wave is past its reads of stage B^1 (each
     // read fed an MFMA already issued); a bare barrier, as a fence here would
     // also wait for the LDS loads still in flight

@@ -78,6 +78,39 @@ def test_grouped_and_graph_match_plain(bat, hip_dev, cfgs):
         b.replay()
 
 
+@pytest.mark.parametrize("cfg", range(7))
+def test_one_configuration_lists_match_plain(bat, hip_dev, cfg):
+    """A batch whose GEMMs all use one tile configuration runs that
+    configuration's kernel over per-workgroup lists built for the launch's
+    grid, each item's first K tile loaded during the previous item's last.
+    It matches the plain launches to the bit at every grid size (fewer
+    workgroups than XCD segments, more than there are items, the occupancy-
+    sized grid) and under graph replay, with eager work between replays."""
+    dims = bat.shapes(n=48, seed=17, lo=16, hi=700) + EDGE_DIMS
+    b = bat.Batch(dims, dtype="bf16", seed=4)
+    b.set_configs([cfg] * len(dims))
+    b.run_plain(-1)
+    hip_dev.synchronize()
+    want = [b.result(i) for i in range(len(dims))]
+    for c in b.C:
+        c.fill_zero()
+    for grid in (97, 3, 0, 100000, 5, 0):
+        b.run_grouped(grid=grid)
+        hip_dev.synchronize()
+        for i in range(len(dims)):
+            assert np.array_equal(b.result(i), want[i]), ("grouped", cfg, grid, i, dims[i])
+        for c in b.C:
+            c.fill_zero()
+    b.capture()
+    for rep in range(3):
+        b.replay()
+        hip_dev.synchronize()
+        for i in range(len(dims)):
+            assert np.array_equal(b.result(i), want[i]), ("replay", cfg, rep, i)
+        for c in b.C:
+            c.fill_zero()
+
+
 def test_tuned_batch_vs_f64(bat, hip_dev, lt, tmp_path):
     """Tune the buckets, store one schedule per bucket, and build a fresh
     batch from the stored schedules: its configurations are the tuned ones,
