@@ -49,6 +49,8 @@ def parse():
                          "128x128x32 tile), for profiling the grouped kernel")
     ap.add_argument("--no-conv", action="store_true",
                     help="softmax run: skip the conv2d bf16 kernel and plan figures")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="softmax run: skip the batched-GEMM graph replay figure")
     return ap.parse_args()
 
 
@@ -617,6 +619,8 @@ def main():
     }
     if args.nest == "softmax" and not args.no_conv:
         line["config"]["conv2d_bf16"] = conv_bf16_figures(L, hip)
+    if args.nest == "softmax" and not args.no_batched:
+        line["config"]["batched_bf16"] = batched_figures(hip)
     if args.nest == "matmul":
         line["config"]["launch_floor"] = launch_floor(L, hip)
         line["config"]["workgroups"] = st_grid(cc)
@@ -626,7 +630,8 @@ def main():
                                             flops_step if bound != "hbm" else 0)
     print(json.dumps(line))
     bad = [k for k, v in [("", errors)] + [
-        ("conv2d_bf16", line["config"].get("conv2d_bf16", {}).get("errors", {"ok": True}))]
+        (k, line["config"].get(k, {}).get("errors", {"ok": True}))
+        for k in ("conv2d_bf16", "batched_bf16")]
         if not v["ok"]]
     if bad:
         print("bench.py: result outside its error bar: %s" % (bad,), file=sys.stderr)
@@ -684,6 +689,40 @@ def conv_bf16_figures(L, hip):
                 kernel_frac=round(kt / peak, 4), plan_ms=round(plan_ms, 5),
                 plan_tflops=round(pt, 2), plan_frac=round(pt / peak, 4), peak_tflops=peak,
                 errors=errors)
+
+
+def batched_figures(hip):
+    """The 512 uneven bf16 GEMMs of `--nest batched` (M, N, K log-uniform in
+    [32, 2048], multiples of 16, seed 6) measured in the same run as the
+    softmax: one graph-replayed launch with the 128x128x64 tile on every GEMM
+    (the tile the tuner picks for every shape bucket of this batch; `--nest
+    batched` runs the tuning itself), the median of three HIP-event timings
+    of 20 replays, its fraction of the HBM roofline, and the error on 48
+    sampled rows of every GEMM."""
+    from loop_tool_amd import batched as B
+    dims = B.shapes(n=512, seed=6)
+    flops = sum(2 * m * n * k for m, n, k in dims)
+    nbytes = sum(2 * (m * k + k * n) + 4 * m * n for m, n, k in dims)
+    stream = hip.Stream()
+    b = B.Batch(dims, dtype="bf16", seed=6, stream=stream.h)
+    b.set_configs([3] * len(dims))
+    b.capture()
+    settle(b.replay, stream.synchronize)
+    ms = sorted(B.time_ms(b.replay, iters=20, stream=stream.h) for _ in range(3))[1]
+    e, e16 = batch_errors(b)
+    items = b.num_items()
+    b.free()
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    tol = {"f64": 1e-2, "f64_rounded": 1e-5}
+    return dict(workload="512 independent matmul nests, M,N,K log-uniform in [32, 2048] "
+                         "(multiples of 16), bf16 operands, fp32 accumulate, one "
+                         "graph-replayed launch, 128x128x64 tile on every GEMM",
+                gemms=len(dims), work_items=items, algorithmic_flops=flops,
+                algorithmic_bytes=nbytes, graph_ms=round(ms, 5),
+                tflops=round(flops / (ms * 1e-3) / 1e12, 2), hbm_gbs=round(gbs, 1),
+                hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
+                errors=dict(f64=e, f64_rounded=e16, rows="48 random rows of every GEMM",
+                            ok=e <= tol["f64"] and e16 <= tol["f64_rounded"], tolerance=tol))
 
 
 if __name__ == "__main__":
