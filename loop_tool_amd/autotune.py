@@ -211,7 +211,7 @@ def knob_candidates(kind):
                 "contract=staged streamk=1", "contract=gather"]
     if kind.startswith("contract"):  # staged 16-bit operands
         return ["stage=flat", "waves=8", "waves=2", "igemm=halo", "igemm=plain",
-                "tile=256x256 bk=64", "tile=128x256"]
+                "tile=256x256 bk=64", "tile=128x256", "srows=2"]
     if kind == "pointwise":
         return ["block=%d vec=%d" % (b, v) for b, v in itertools.product((256,), (4, 1))]
     return [""]

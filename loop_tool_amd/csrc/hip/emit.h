@@ -76,6 +76,9 @@ struct Knobs {
   int ksplit = -1;             // "ksplit=0|1": gathered contractions on 32x32
                                // tiles with K tiles split over the waves
                                // (-1 = auto: nests under 64 64x64 tiles)
+  int stage_rows = 0;          // "srows=N": rows of a source plane per workgroup
+                               // of the plane-staging pass (0 = auto: 256
+                               // positions' worth)
   int igemm_halo = -1;         // "igemm=halo" (1) / "igemm=halo1" (2, one halo
                                // buffer) / "igemm=pipe" (0): staged 16-bit
                                // contractions load the Q rows of all taps of a

@@ -182,14 +182,17 @@ void fixed_dims(std::stringstream &k, const Operand &op, const Staged &s,
 // instruction hit distinct banks), then writes every staged position of the
 // rows, halo included, as 16-byte chunks of TI/8 consecutive inner values.
 int64_t staging_plane_body(std::stringstream &k, const std::string &name, const Operand &op,
-                           const Staged &s, int ld, int a, int64_t kin, const char *elt) {
+                           const Staged &s, int ld, int a, int64_t kin, const char *elt,
+                           int rows) {
   const int nd = (int)op.sizes.size(), in = s.inner_dim;
   // inner values per workgroup: 64 16-bit or 32 float values (the LDS tile
   // holds NP positions of TI values)
   const bool wide = std::string(elt) == "float";
   const int TI = s.ssize[in] % 64 == 0 && !wide ? 64 : 32, TP = TI + 4;
   const int64_t L = op.sizes[ld];
-  const int64_t RA = std::min<int64_t>(s.ssize[a], std::max<int64_t>(1, 256 / L));
+  // rows per workgroup: 256 positions' worth unless given, within 64 KB of LDS
+  int64_t RA = rows > 0 ? rows : std::max<int64_t>(1, 256 / L);
+  RA = std::max<int64_t>(1, std::min<int64_t>({RA, s.ssize[a], 65536 / (L * TP * (wide ? 4 : 2))}));
   const int64_t NP = RA * L, NQ = (NP + 3) / 4, NQP = (NQ + 7) / 8 * 8, CP = TI / 2;
   const int64_t nA = (s.ssize[a] + RA - 1) / RA, nC = s.ssize[in] / TI;
   std::vector<int> other;
@@ -379,7 +382,7 @@ int64_t staging_generic_body(std::stringstream &k, const std::string &name, cons
 // Emits the staging of one operand as `__device__ void name(in, S, bid)` and
 // returns its workgroup count; picks the plane, transpose or generic form.
 int64_t staging_body(std::stringstream &k, const std::string &name, const Operand &op,
-                     const Staged &s, int64_t kin, const char *elt) {
+                     const Staged &s, int64_t kin, const char *elt, int rows) {
   const int nd = (int)op.sizes.size(), in = s.inner_dim;
   const int ld = source_contiguous_dim(op, s);
   if (ld < 0 || op.strides[ld] != 1 || s.ssize[in] % 32 != 0)
@@ -387,7 +390,7 @@ int64_t staging_body(std::stringstream &k, const std::string &name, const Operan
   for (int a = 0; a < nd; ++a)
     if (a != ld && a != in && op.strides[a] == op.sizes[ld] && s.ssize[a] > 1 &&
         op.sizes[ld] <= 512)
-      return staging_plane_body(k, name, op, s, ld, a, kin, elt);
+      return staging_plane_body(k, name, op, s, ld, a, kin, elt, rows);
   if (s.ssize[ld] >= 32) return staging_transpose_body(k, name, op, s, ld, kin, elt);
   return staging_generic_body(k, name, op, s, kin, elt);
 }
@@ -709,7 +712,8 @@ bool try_contract_staged(const Dataflow &df, const Contraction &c, const Knobs &
   // operand 0, the rest operand 1 ----
   int64_t nb[2];
   for (int o = 0; o < 2; ++o) {
-    nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt);
+    nb[o] = staging_body(k, "lt_stage_op" + std::to_string(o), c.ops[o], st[o], KI, elt,
+                         knobs.stage_rows);
     if (o == c.qo && qdirect) nb[o] = 0;  // the tile reads Q's source
   }
   ASSERT(nb[0] + nb[1] + nzero < (1LL << 31)) << "staging grid too large";

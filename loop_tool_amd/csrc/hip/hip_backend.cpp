@@ -21,6 +21,8 @@
 //   vec=N          elements per lane for row reductions
 //   tile=MxN       workgroup tile of contractions
 //   bk=N           K tile (32 or 64) of 16-bit contractions
+//   srows=N        rows of a source plane per workgroup when contraction
+//                  operands are staged through LDS plane tiles
 //   contract=gather  contractions gather their operands in the tile loader
 //                  instead of staging them
 //   halo=direct|staged  the halo tile reads the activations' float source
@@ -301,6 +303,10 @@ Knobs knobs_from(const LoopTree &lt, bool bf16_variant) {
         else bad();
       } else if (key == "waves") {
         k.waves = number(tok, v);
+      } else if (key == "srows") {
+        k.stage_rows = number(tok, v);
+        ASSERT(k.stage_rows >= 1 && k.stage_rows <= 64)
+            << "knob " << tok << ": srows=N must be in [1, 64]";
       } else if (key == "bk") {
         k.bk = number(tok, v);
         ASSERT(k.bk == 32 || k.bk == 64) << "knob " << tok << ": bk=N must be 32 or 64";

@@ -101,6 +101,7 @@ def test_streamk_plans(lt):
                                       ("vec=3", "vec=N must be 1, 2 or 4"),
                                       ("igemm=foo", "unknown value"),
                                       ("streamk=2", "unknown value"),
+                                      ("srows=0", "srows=N must be in \\[1, 64\\]"),
                                       ("waves=-1", "expected a number")])
 def test_bad_knob_values_are_errors(lt, knob, msg):
     """A token whose key is a knob of the hip backend must carry a valid

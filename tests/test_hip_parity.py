@@ -258,6 +258,25 @@ def test_conv_bf16_halo_tile(lt, hip_dev, name, kw, tile):
     assert normwise(ys[0], f64_bf) <= 1e-5
 
 
+@pytest.mark.parametrize("name,kw", HALO, ids=["%s%d" % (c[0], i) for i, c in enumerate(HALO)])
+def test_plane_staging_rows_bit_identical(lt, hip_dev, name, kw):
+    """srows=N sets how many rows of a source plane one workgroup of the
+    plane-staging pass handles; the staged operands, and so the results, are
+    the same bits for every N."""
+    out, shapes = getattr(P, name)(lt, **kw)
+    inputs = contract_inputs(name, shapes)
+    ys = []
+    for knobs in ("", "srows=1", "srows=3", "srows=64"):
+        t = out.loop_tree
+        if knobs:
+            t = t.annotate(t.roots[0], knobs)
+        y = np.zeros(t.io_numel()[len(inputs)], np.float32)
+        lt.compile(t, "hip_bf16").run(inputs + [y])
+        ys.append(y)
+    for y in ys[1:]:
+        assert np.array_equal(y.view(np.uint32), ys[0].view(np.uint32))
+
+
 @pytest.mark.parametrize("name,kw", CONTRACT, ids=["%s%d" % (c[0], i) for i, c in enumerate(CONTRACT)])
 @pytest.mark.parametrize("backend", ["hip", "hip_bf16"])
 def test_ksplit_tile_vs_whole_tile(lt, hip_dev, name, kw, backend):
