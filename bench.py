@@ -522,6 +522,8 @@ def main():
         return bench_batched(args)
     if args.nest == "serving":
         return bench_serving(args)
+    if args.nest == "pack":
+        return bench_pack(args)
     import loop_tool_amd as lt_mod
     from loop_tool_amd import hip
 
@@ -689,6 +691,104 @@ def conv_bf16_figures(L, hip):
                 kernel_frac=round(kt / peak, 4), plan_ms=round(plan_ms, 5),
                 plan_tflops=round(pt, 2), plan_frac=round(pt / peak, 4), peak_tflops=peak,
                 errors=errors)
+
+
+def bench_pack(args):
+    """64 nests of the serving mix (loop_tool_amd/packed.py: matmul, row
+    softmax, 3x3 conv, 2x2 max-pool of random sizes, seed 7), each compiled by
+    the hip backend with its own device inputs, output and scratch.  A step is
+    one pass over all of them: as one graph replay of a pack over 8 streams
+    (lth_pack_*).  Beside it, the same pass as one launch sequence on one
+    stream (lth_run_on per nest) and as packs over 1 and 4 streams; every
+    form's outputs are compared with the launch sequence's, bit for bit."""
+    import loop_tool_amd as lt_mod
+    from loop_tool_amd import hip, packed
+    hip.init()
+    L = lt_mod.lib
+    specs = packed.serving_mix(n=64, seed=7)
+    t0 = time.perf_counter()
+    compiled = packed.compile_nests(L, specs)
+    compile_s = time.perf_counter() - t0
+    rng = np.random.default_rng(7)
+    nests, scratch = [], []
+    for cc, tree, shapes in compiled:
+        d_in = [hip.DeviceArray.from_numpy(rng.standard_normal(int(np.prod(sh)), np.float32))
+                for sh in shapes]
+        d_out = hip.DeviceArray(tree.io_numel()[len(shapes)])
+        nests.append((cc, d_in + [d_out]))
+        scratch.append([hip.DeviceArray(max(w, 1)) for w in hip.scratch_sizes(cc)])
+    flops = sum(sum(st["flops"] for st in hip.stages(cc)) for cc, _ in nests)
+    nbytes = sum(sum(st["bytes"] for st in hip.stages(cc)) for cc, _ in nests)
+    stream = hip.Stream()
+
+    def sequence():
+        for (cc, io), sc in zip(nests, scratch):
+            hip.run_on(cc, io, sc, stream.h)
+
+    def timed(fn, iters=10):
+        fn()
+        stream.synchronize()
+        settle(fn, stream.synchronize)
+        e0, e1 = hip.Event(), hip.Event()
+        best = []
+        for _ in range(3):
+            e0.record(stream.h)
+            for _ in range(iters):
+                fn()
+            e1.record(stream.h)
+            e1.synchronize()
+            best.append(e0.elapsed_ms(e1) / iters)
+        return sorted(best)[1]
+
+    seq_ms = timed(sequence)
+    want = [io[-1].download() for _, io in nests]
+    forms, same = {}, {}
+    for lanes in (1, 4, 8):
+        for _, io in nests:
+            io[-1].fill_zero()
+        hip.synchronize()
+        pk = hip.Pack(nests, lanes=lanes, stream=stream.h)
+        forms[lanes] = timed(pk.replay)
+        same[lanes] = all(np.array_equal(io[-1].download().view(np.uint32), w.view(np.uint32))
+                          for (_, io), w in zip(nests, want))
+        if lanes == 8:
+            for _ in range(args.warmup):
+                pk.replay()
+            stream.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                pk.replay()
+            stream.synchronize()
+            el = time.perf_counter() - t1
+        pk.free()
+    counts = {}
+    for name, _, _ in specs:
+        counts[name] = counts.get(name, 0) + 1
+    line = {
+        "metric": "mixed nests per second (one graph replay of a pack), hip backend",
+        "value": round(len(nests) * args.steps / el, 1),
+        "unit": "nests/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16+f32",
+        "data": "synthetic (numpy normal inputs, seeded sizes)",
+        "config": dict(workload="64 nests of the serving mix in one pass", nests=counts,
+                       algorithmic_flops=flops, algorithmic_bytes=nbytes,
+                       compile_s=round(compile_s, 1),
+                       one_stream_sequence_ms=round(seq_ms, 4),
+                       pack_ms={str(k): round(v, 4) for k, v in forms.items()},
+                       speedup_vs_sequence={str(k): round(seq_ms / v, 3) for k, v in forms.items()},
+                       bit_identical_to_sequence={str(k): v for k, v in same.items()}),
+    }
+    print(json.dumps(line))
+    if not all(same.values()):
+        print("bench.py: a pack's outputs differ from the launch sequence's", file=sys.stderr)
+        sys.exit(3)
 
 
 def batched_figures(hip):

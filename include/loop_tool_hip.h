@@ -80,6 +80,24 @@ ltc_looptree *lth_tuned_get(const ltc_looptree *program);
 #define LTH_STATIC_SCRATCH_BYTES (64u << 20)
 int lth_run_static(const ltc_compiled *cc, void *const *io, int n_io, void *stream);
 
+/* ---- packs of compiled nests in one graph.  No counterpart in the
+ * reference, which runs one Compiled at a time (backend.h:18-22).  Any hip
+ * Compiled plans (pointwise, row reductions, contractions, convolutions,
+ * several stages each) with their device buffers (inputs then outputs of
+ * each nest, concatenated: n_io[i] pointers for nest i) are captured into
+ * one hipGraph: the pack allocates every nest's scratch once, spreads the
+ * nests over `lanes` (1..16) streams forked from `stream` and joined back
+ * (longest first, each to the lane with the least estimated time, from the
+ * plans' algorithmic bytes and flops), so independent nests run
+ * concurrently, and a replay is one graph launch.  The nests must not write
+ * a buffer another nest of the pack reads or writes.  `stream` must be a
+ * created stream; replays of one pack must not overlap. */
+typedef struct lth_pack lth_pack;
+lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const int *n_io,
+                          int n, int lanes, void *stream);
+int lth_pack_replay(lth_pack *p, void *stream);
+void lth_pack_free(lth_pack *p);
+
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
  * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
  * transposed Bt[N][K]; tile configurations 0..5 are 64x64, 128x64, 64x128,

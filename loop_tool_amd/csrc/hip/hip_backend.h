@@ -44,8 +44,13 @@ class HipCompiled : public Compiled {
   // one kernel).
   void run_stage(const std::vector<void *> &device_io, int k, void *stream) const;
 
+  // load the plan's code object now (before a graph capture, say)
+  void prepare() const {
+    std::lock_guard<std::mutex> g(mu_);
+    load();
+  }
   // true once the plan's code object is loaded on the device (by any run,
-  // run_on, run_stage or stream())
+  // run_on, run_stage, stream() or prepare())
   bool loaded() const {
     std::lock_guard<std::mutex> g(mu_);
     return loaded_;

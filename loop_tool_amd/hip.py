@@ -246,3 +246,48 @@ def time_stage_ms(compiled, io, stage, warmup=3, iters=20):
     _lth(_L.lth_time_stage(compiled.h, _ptrs(io), len(io), stage, warmup, iters,
                            C.byref(ms)))
     return ms.value
+
+
+_sig("lth_pack_create", _P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int), C.c_int, C.c_int,
+     _P)
+_sig("lth_pack_replay", C.c_int, _P, _P)
+_L.lth_pack_free.restype = None
+_L.lth_pack_free.argtypes = [_P]
+
+
+class Pack:
+    """Many compiled nests captured into one hipGraph (include/loop_tool_hip.h,
+    lth_pack_*): `nests` is a list of (Compiled, [device buffers, inputs then
+    outputs]); the nests are spread over `lanes` streams that run
+    concurrently; `replay()` runs all of them with one graph launch on
+    `stream` (a created Stream's handle, by default the pack's own)."""
+
+    def __init__(self, nests, lanes=4, stream=None):
+        if stream is None:
+            self._own_stream = Stream()
+            stream = self._own_stream.h
+        self.stream = stream
+        self._keep = [c for c, _ in nests]  # Compiled handles outlive the graph
+        cc = (_P * len(nests))(*[c.h for c, _ in nests])
+        flat = [b for _, io in nests for b in io]
+        n_io = (C.c_int * len(nests))(*[len(io) for _, io in nests])
+        self.h = _L.lth_pack_create(cc, _ptrs(flat), n_io, len(nests), lanes, stream)
+        if not self.h:
+            raise HipError(_L.lth_last_error().decode())
+
+    def replay(self):
+        _lth(_L.lth_pack_replay(self.h, self.stream))
+
+    def synchronize(self):
+        _check(_L.lt_hip_stream_synchronize(self.stream))
+
+    def free(self):
+        if getattr(self, "h", None):
+            _L.lth_pack_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

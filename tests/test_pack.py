@@ -1,0 +1,80 @@
+"""Packs of compiled nests in one hipGraph (include/loop_tool_hip.h,
+lth_pack_*).
+
+Nests of every plan kind -- pointwise, index-only, row reductions, gathered
+and staged contractions in float32 and bf16, convolutions -- are captured into
+one graph, spread over one or several concurrent streams.  Each replay must
+give every nest's output bit for bit as one Compiled::run of that nest does,
+over repeated replays with eager work (the outputs cleared) between them.
+"""
+import numpy as np
+import pytest
+
+from loop_tool_amd import programs as P
+
+pytestmark = pytest.mark.gpu
+
+MIX = [
+    ("softmax", dict(R=64, Cn=1000), "hip"),
+    ("softmax", dict(R=300, Cn=4096), "hip"),
+    ("matmul", dict(M=128, N=128, K=128), "hip"),
+    ("matmul", dict(M=300, N=200, K=704), "hip_bf16"),
+    ("matmul", dict(M=37, N=53, K=29), "hip"),
+    ("conv2d", dict(N=2, Ci=32, Co=40, H=14, W=14), "hip_bf16"),
+    ("conv2d", dict(N=2, Ci=8, Co=16, H=12, W=12), "hip"),
+    ("maxpool2d", dict(N=2, C=3, H=16, W=16), "hip"),
+    ("pad2d", dict(N=2, C=5, H=9, W=11), "hip"),
+    ("concat2d", dict(B=17, M0=33, M1=20), "hip"),
+    ("transpose2d", dict(M=300, N=77), "hip"),
+    ("add_broadcast", dict(M=513, N=129), "hip"),
+]
+
+
+def build(lt, hip_dev):
+    """(Compiled, device buffers) per nest of the mix, each with its own
+    seeded inputs and an output buffer."""
+    nests = []
+    for i, (name, kw, backend) in enumerate(MIX):
+        out, shapes = getattr(P, name)(lt, **kw)
+        tree = out.loop_tree
+        inputs = [P.normal(100 + 7 * i + j, s) for j, s in enumerate(shapes)]
+        cc = lt.compile(tree, backend)
+        d_in = [hip_dev.DeviceArray.from_numpy(a) for a in inputs]
+        d_out = hip_dev.DeviceArray(tree.io_numel()[len(inputs)])
+        nests.append((cc, d_in + [d_out]))
+    return nests
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_pack_replays_match_each_run(lt, hip_dev, lanes):
+    nests = build(lt, hip_dev)
+    want = []
+    for cc, io in nests:
+        cc.run([b.ptr for b in io])
+        want.append(io[-1].download())
+    for _, io in nests:
+        io[-1].fill_zero()
+    hip_dev.synchronize()
+    pack = hip_dev.Pack(nests, lanes=lanes)
+    try:
+        for rep in range(3):
+            pack.replay()
+            pack.synchronize()
+            for i, (_, io) in enumerate(nests):
+                got = io[-1].download()
+                assert np.array_equal(got.view(np.uint32), want[i].view(np.uint32)), (
+                    rep, MIX[i])
+            for _, io in nests:  # eager work between replays
+                io[-1].fill_zero()
+            hip_dev.synchronize()
+    finally:
+        pack.free()
+
+
+def test_pack_refuses_bad_arguments(lt, hip_dev):
+    nests = build(lt, hip_dev)[:2]
+    with pytest.raises(Exception, match="lanes"):
+        hip_dev.Pack(nests, lanes=0)
+    cc, io = nests[0]
+    with pytest.raises(Exception, match="device buffers"):
+        hip_dev.Pack([(cc, io[:-1])], lanes=1)
