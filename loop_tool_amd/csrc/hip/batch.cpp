@@ -21,6 +21,7 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <queue>
@@ -116,10 +117,20 @@ struct lth_batch {
   // the work list in segment order on the host, and its segment offsets
   std::vector<Item> h_items;
   std::vector<int> h_xoff;
-  // per-workgroup lists of the one-configuration kernels, for lists_grid
-  // workgroups (0: none built for the current work list)
-  void *d_sitems = nullptr, *d_lists = nullptr;
-  int lists_grid = 0;
+  // per-workgroup lists of the one-configuration kernels, one set per grid
+  // size launched since the work list was built (a graph captured with one
+  // grid keeps reading its set while launches with other grids run)
+  struct Lists {
+    void *items = nullptr, *offsets = nullptr;
+  };
+  std::map<int, Lists> lists;
+  void free_lists() {
+    for (auto &kv : lists) {
+      if (kv.second.items) lt_hip_free(kv.second.items);
+      if (kv.second.offsets) lt_hip_free(kv.second.offsets);
+    }
+    lists.clear();
+  }
   bool dirty = true;
   void *graph = nullptr;
   void *graph_stream = nullptr;
@@ -197,7 +208,7 @@ struct lth_batch {
     n_items = (int)sorted.size();
     h_items = sorted;
     h_xoff = xoff;
-    lists_grid = 0;
+    free_lists();  // (the graph that read them is dropped below)
     dirty = false;
     if (graph) {
       lt_hip_graph_destroy(graph);
@@ -235,8 +246,9 @@ struct lth_batch {
   // cost of kItemFixed plus its K tiles; a list keeps the segment's order
   // (decreasing cost).  Uploaded with blocking copies: built before a graph
   // capture, never during one.
-  void static_lists(int G) {
-    if (lists_grid == G) return;
+  const Lists &static_lists(int G) {
+    auto found = lists.find(G);
+    if (found != lists.end()) return found->second;
     std::vector<std::vector<Item>> per(G);
     typedef std::pair<double, int> Load;
     for (int x = 0; x < kXcd; ++x) {
@@ -260,14 +272,16 @@ struct lth_batch {
       flat.insert(flat.end(), per[w].begin(), per[w].end());
       off[w + 1] = (int)flat.size();
     }
-    if (d_sitems) lt_hip_free(d_sitems);
-    if (d_lists) lt_hip_free(d_lists);
-    chk(lt_hip_malloc(&d_sitems, std::max<size_t>(flat.size(), 1) * sizeof(Item)), "lists");
-    chk(lt_hip_malloc(&d_lists, off.size() * sizeof(int)), "list offsets");
+    Lists l;
+    chk(lt_hip_malloc(&l.items, std::max<size_t>(flat.size(), 1) * sizeof(Item)), "lists");
+    lists[G] = l;  // freed with the others if a copy below fails
+    chk(lt_hip_malloc(&lists[G].offsets, off.size() * sizeof(int)), "list offsets");
     if (!flat.empty())
-      chk(lt_hip_memcpy(d_sitems, flat.data(), flat.size() * sizeof(Item), 0), "lists copy");
-    chk(lt_hip_memcpy(d_lists, off.data(), off.size() * sizeof(int), 0), "offsets copy");
-    lists_grid = G;
+      chk(lt_hip_memcpy(lists[G].items, flat.data(), flat.size() * sizeof(Item), 0),
+          "lists copy");
+    chk(lt_hip_memcpy(lists[G].offsets, off.data(), off.size() * sizeof(int), 0),
+        "offsets copy");
+    return lists[G];
   }
 
   // the workgroups of a grouped launch: grid <= 0 means as many as the GPU
@@ -292,7 +306,12 @@ struct lth_batch {
     unsigned lds = 0;
     void *fn = grouped_kernel(&lds);
     const bool one = one_config();
-    void *a0 = one ? d_sitems : d_items, *a1 = one ? d_lists : d_queue;
+    void *a0 = d_items, *a1 = d_queue;
+    if (one) {
+      const Lists &l = static_lists(grid);
+      a0 = l.items;
+      a1 = l.offsets;
+    }
     void *args[] = {&a0, &a1};
     chk(lt_hip_launch(fn, grid, 1, 1, 256, 1, 1, lds, stream, args), "grouped launch");
   }
@@ -318,8 +337,7 @@ struct lth_batch {
     if (graph) lt_hip_graph_destroy(graph);
     if (d_items) lt_hip_free(d_items);
     if (d_queue) lt_hip_free(d_queue);
-    if (d_sitems) lt_hip_free(d_sitems);
-    if (d_lists) lt_hip_free(d_lists);
+    free_lists();
   }
 };
 
