@@ -71,7 +71,9 @@ class HipCompiled : public Compiled {
   // (two sets with overlap), device copies of host buffers and the overlap
   // stream and events.  Created on the thread's first call and used by that
   // thread only, so runs from different threads proceed concurrently, each
-  // in order on its own stream.
+  // in order on its own stream.  A context lives as long as the Compiled
+  // (threads that end leave theirs): one stream and one scratch set per
+  // thread that ever ran it.
   struct RunContext {
     void *stream = nullptr;
     std::vector<void *> scratch, scratch2;
