@@ -45,7 +45,7 @@ def build(lt, hip_dev):
     return nests
 
 
-@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("lanes", [1, 4, 8])
 def test_pack_replays_match_each_run(lt, hip_dev, lanes):
     nests = build(lt, hip_dev)
     want = []
@@ -78,3 +78,33 @@ def test_pack_refuses_bad_arguments(lt, hip_dev):
     cc, io = nests[0]
     with pytest.raises(Exception, match="device buffers"):
         hip_dev.Pack([(cc, io[:-1])], lanes=1)
+
+
+def test_pack_runs_one_compiled_on_several_buffer_sets(lt, hip_dev):
+    """One Compiled packed three times over different inputs and outputs
+    (each entry gets scratch of its own): every output is its own run's."""
+    out, shapes = P.conv2d(lt, N=2, Ci=32, Co=40, H=14, W=14)
+    tree = out.loop_tree
+    cc = lt.compile(tree, "hip_bf16")
+    nests, want = [], []
+    for r in range(3):
+        d_in = [hip_dev.DeviceArray.from_numpy(P.normal(500 + 10 * r + j, s))
+                for j, s in enumerate(shapes)]
+        d_out = hip_dev.DeviceArray(tree.io_numel()[len(shapes)])
+        cc.run([b.ptr for b in d_in + [d_out]])
+        want.append(d_out.download())
+        d_out.fill_zero()
+        nests.append((cc, d_in + [d_out]))
+    hip_dev.synchronize()
+    pack = hip_dev.Pack(nests, lanes=3)
+    try:
+        for rep in range(2):
+            pack.replay()
+            pack.synchronize()
+            for i, (_, io) in enumerate(nests):
+                assert np.array_equal(io[-1].download().view(np.uint32),
+                                      want[i].view(np.uint32)), (rep, i)
+                io[-1].fill_zero()
+            hip_dev.synchronize()
+    finally:
+        pack.free()
