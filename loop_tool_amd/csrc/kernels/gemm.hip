@@ -206,10 +206,8 @@ __device__ __forceinline__ void lt_ggemm_static(const LtWorkItem* __restrict__ i
     for (int kt = 0; kt < cur.KT; ++kt) {
       const int B = (st + kt) & 1;
       // this wave's loads of tile kt have landed (and the previous item's
-      // stores); then everyone's, and every wave is past its reads of the
-      // other stage (a bare barrier: see lt_igemm_tile_dl)
-      __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(0));
-      __builtin_amdgcn_s_barrier();
+      // stores) and its reads of the other stage are done; then everyone's
+      lt_ig_lds_barrier<lt_ig_vmcnt(0)>();
       if (kt == 0 && more) nit = items[nidx];  // the next item's record
       const bool last = kt + 1 == cur.KT;
       if (last && more) nxt = job(nit);

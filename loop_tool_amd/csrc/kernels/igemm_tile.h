@@ -407,6 +407,23 @@ __host__ __device__ constexpr int lt_ig_vmcnt(int n) {
   return (n & 15) | (((n >> 4) & 3) << 14) | (0x7 << 4) | (0xF << 8);
 }
 
+// Workgroup barrier of the tiles whose LDS stages are filled by
+// direct-to-LDS loads.  Before it, this wave's LDS reads have completed
+// (lgkmcnt 0) and its loads up to WAIT (an lt_ig_vmcnt operand) have landed.
+// The compiler fences on both sides keep LDS reads on their side: the
+// s_barrier builtin alone is no memory barrier to the compiler, which may
+// hoist the next K tile's fragment reads above it (reading a stage before
+// another wave's load into it has landed) or leave reads in flight past it
+// (while another wave's next load rewrites their stage); either gives wrong
+// fragments when the loads are slow, e.g. with other kernels on the CU.
+template <int WAIT>
+__device__ __forceinline__ void lt_ig_lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(WAIT & ~(0xF << 8));  // and lgkmcnt(0)
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 template <typename T, int BM, int BN, int BK, bool FULL, int WGM, int WGN, bool KTAIL = false,
           class G>
 __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict__ P,
@@ -484,11 +501,9 @@ __device__ __forceinline__ void lt_igemm_tile_dl(const G& g, const T* __restrict
 #pragma unroll
     for (int b = 0; b < TN; ++b) acc[a][b] = (lt_ig_f32x16){};
   auto step = [&](int kt, int B) {
-    __builtin_amdgcn_s_waitcnt(lt_ig_vmcnt(0));  // this wave's loads of tile kt have landed
-    // everyone's have, and every wave is past its reads of stage B^1 (each
-    // read fed an MFMA already issued); a bare barrier, as a fence here would
-    // also wait for the LDS loads still in flight
-    __builtin_amdgcn_s_barrier();
+    // this wave's loads of tile kt have landed and its reads of stage B^1
+    // are done; then everyone's
+    lt_ig_lds_barrier<lt_ig_vmcnt(0)>();
     const bool more = kt + 1 < KT;
     frags(B, 0, 0);
 #pragma unroll
@@ -976,12 +991,10 @@ __device__ __forceinline__ void lt_igemm_tile_halo_dl(const G& g, const T* __res
   auto step = [&](int kt, int B) {
     const int kb = kt / NTAP, tap = kt - kb * NTAP;
     const bool next_halo = kb + 1 < KB;
-    // the top of a block's second tap leaves the next block's halo in flight
-    if (tap == 1 && next_halo) __builtin_amdgcn_s_waitcnt(kWaitVmHV);
-    else __builtin_amdgcn_s_waitcnt(kWaitVm0);
-    // tile kt's P and block kb's halo are in LDS, stage B^1 is free (see
-    // lt_igemm_tile_dl: a bare barrier keeps the next halo's loads in flight)
-    __builtin_amdgcn_s_barrier();
+    // tile kt's P and block kb's halo are in LDS, stage B^1 is free; the top
+    // of a block's second tap leaves the next block's halo in flight
+    if (tap == 1 && next_halo) lt_ig_lds_barrier<kWaitVmHV>();
+    else lt_ig_lds_barrier<kWaitVm0>();
     const bool more = kt + 1 < KT;
     const int hb = kb & 1, sh = shift(kt);
     frags(B, hb, sh, 0, 0);
