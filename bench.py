@@ -761,6 +761,28 @@ def bench_pack(args):
             stream.synchronize()
             el = time.perf_counter() - t1
         pk.free()
+    # the same packs with the plain 16-bit matmul nests grouped into one
+    # persistent GEMM launch: those within 1e-5 (normwise) of the sequence,
+    # the rest bit for bit
+    grouped = [cc.int_property("plain_matmul") == 1 and
+               hip.stages(cc)[-1]["kind"] in ("contract_bf16", "contract_f16")
+               for cc, _ in nests]
+    gforms, gerr, gsame = {}, 0.0, True
+    for lanes in (1, 8):
+        for _, io in nests:
+            io[-1].fill_zero()
+        hip.synchronize()
+        pk = hip.Pack(nests, lanes=lanes, stream=stream.h, group_matmuls=True)
+        gforms[lanes] = timed(pk.replay)
+        for (_, io), w, g in zip(nests, want, grouped):
+            got = io[-1].download()
+            if g:
+                w64 = w.astype(np.float64)
+                gerr = max(gerr, float(np.linalg.norm(got.astype(np.float64) - w64) /
+                                       max(np.linalg.norm(w64), 1e-30)))
+            else:
+                gsame = gsame and np.array_equal(got.view(np.uint32), w.view(np.uint32))
+        pk.free()
     counts = {}
     for name, _, _ in specs:
         counts[name] = counts.get(name, 0) + 1
@@ -783,10 +805,15 @@ def bench_pack(args):
                        one_stream_sequence_ms=round(seq_ms, 4),
                        pack_ms={str(k): round(v, 4) for k, v in forms.items()},
                        speedup_vs_sequence={str(k): round(seq_ms / v, 3) for k, v in forms.items()},
-                       bit_identical_to_sequence={str(k): v for k, v in same.items()}),
+                       bit_identical_to_sequence={str(k): v for k, v in same.items()},
+                       grouped_matmuls=dict(
+                           nests=int(sum(grouped)),
+                           pack_ms={str(k): round(v, 4) for k, v in gforms.items()},
+                           max_normwise_err_vs_sequence=gerr, tolerance=1e-5,
+                           other_nests_bit_identical=gsame)),
     }
     print(json.dumps(line))
-    if not all(same.values()):
+    if not all(same.values()) or not gsame or gerr > 1e-5:
         print("bench.py: a pack's outputs differ from the launch sequence's", file=sys.stderr)
         sys.exit(3)
 

@@ -91,17 +91,26 @@ int lth_run_static(const ltc_compiled *cc, void *const *io, int n_io, void *stre
  * plans' algorithmic bytes and flops), so independent nests run
  * concurrently, and a replay is one graph launch.  The nests must not write
  * a buffer another nest of the pack reads or writes.  `stream` must be a
- * created stream; replays of one pack must not overlap. */
+ * created stream; replays of one pack must not overlap.
+ * With LTH_PACK_GROUP_MATMULS in `flags`, the nests that are plain 16-bit
+ * matmuls (C[M][N] = A[M][K] B[K][N] over row-major float buffers, compiled
+ * with bf16 or fp16 operands, K a multiple of 8) run instead as their
+ * operands' conversions plus one persistent grouped launch per operand type
+ * (the lth_batch kernels, 128x128x64 tiles): the same operand rounding, but
+ * the grouped tiles' accumulation order, so their outputs agree with each
+ * nest's own run to float32 rounding rather than bit for bit. */
+#define LTH_PACK_GROUP_MATMULS 1
 typedef struct lth_pack lth_pack;
 lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const int *n_io,
-                          int n, int lanes, void *stream);
+                          int n, int lanes, int flags, void *stream);
 int lth_pack_replay(lth_pack *p, void *stream);
 void lth_pack_free(lth_pack *p);
 
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
  * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
- * transposed Bt[N][K]; tile configurations 0..5 are 64x64, 128x64, 64x128,
- * 128x128, 256x128, 128x256 (rows x cols of C). */
+ * transposed Bt[N][K]; tile configurations 0..6 are 64x64x32, 128x64x32,
+ * 64x128x32, 128x128x64, 256x128x32, 128x256x32 and 128x128x32 (rows x cols
+ * of C x K tile). */
 typedef struct lth_batch lth_batch;
 const char *lth_batch_last_error(void);
 /* dtype: 0 bf16, 1 fp16; K must be a multiple of 8 */
@@ -118,6 +127,11 @@ int lth_batch_run_plain(lth_batch *b, int cfg, void *stream);
  * tiles are handed out through a counter owned by the batch, so grouped
  * launches (and graph replays) of one batch must not run concurrently. */
 int lth_batch_run_grouped(lth_batch *b, int grid, void *stream);
+/* resolve the grid of a grouped launch (grid <= 0: the occupancy-sized one)
+ * and upload everything that launch reads; returns that grid (negative on
+ * error).  A grouped launch with the returned grid then allocates and copies
+ * nothing, so a caller can capture it into a graph of its own. */
+int lth_batch_prepare(lth_batch *b, int grid);
 /* capture the grouped launch into a hipGraph on `stream`, replay it later */
 int lth_batch_capture(lth_batch *b, int grid, void *stream);
 int lth_batch_replay(lth_batch *b, void *stream);

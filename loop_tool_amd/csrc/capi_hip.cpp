@@ -197,14 +197,32 @@ struct lth_pack {
     std::vector<void *> io, scratch;
     double est = 0;  // estimated seconds: the plan's roofline time
   };
+  // LTH_PACK_GROUP_MATMULS: the plain 16-bit matmul nests of one operand
+  // type, run as their operands' conversions and one grouped launch
+  struct Cvt {
+    const float *src;
+    void *dst;
+    int rows, cols, transpose;
+  };
+  struct Group {
+    lth_batch *b = nullptr;
+    int dtype = 0, grid = 0;
+    std::vector<Cvt> cvts;
+    double est = 0;
+  };
   std::vector<Nest> nests;
+  std::vector<Group> groups;
+  std::vector<void *> staged;           // the groups' 16-bit operands
   std::vector<void *> streams, events;  // extra lanes; fork and join events
   void *graph = nullptr;
   ~lth_pack() {
     if (loop_tool::hip::runtime_exiting()) return;
     if (graph) lt_hip_graph_destroy(graph);
+    for (auto &g : groups)
+      if (g.b) lth_batch_free(g.b);
+    for (auto q : staged) lt_hip_free(q);
     for (auto &n : nests)
-      for (auto p : n.scratch) lt_hip_free(p);
+      for (auto q : n.scratch) lt_hip_free(q);
     for (auto e : events) lt_hip_event_destroy(e);
     for (auto st : streams) lt_hip_stream_destroy(st);
   }
@@ -213,13 +231,14 @@ struct lth_pack {
 extern "C" {
 
 lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const int *n_io,
-                          int n, int lanes, void *stream) {
+                          int n, int lanes, int flags, void *stream) {
   auto *p = new lth_pack();
   try {
     g_err.clear();
     ASSERT(n > 0 && cc && io && n_io) << "empty pack";
     ASSERT(lanes >= 1 && lanes <= 16) << "lanes must be in [1, 16]";
     ASSERT(stream) << "graph capture needs a created stream (not the null stream)";
+    ASSERT((flags & ~LTH_PACK_GROUP_MATMULS) == 0) << "unknown pack flags " << flags;
     auto chk = [](int rc, const char *what) {
       if (rc) throw std::runtime_error(std::string(what) + ": " + lt_hip_last_error());
     };
@@ -243,17 +262,63 @@ lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const 
                           nest.h->algorithmic_flops() / (f32 ? 157.3e12 : 2.5e15));
       p->nests.push_back(std::move(nest));
     }
-    // lanes: longest first, each nest to the lane with the least estimated time
-    std::vector<int> order(n);
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return p->nests[a].est > p->nests[b].est; });
+    // grouped matmul nests: 16-bit copies of their operands (A[M][K] and the
+    // transposed Bt[N][K]) and one batch per operand type, made ready for a
+    // launch that allocates and copies nothing
+    std::vector<char> grouped(n, 0);
+    if (flags & LTH_PACK_GROUP_MATMULS) {
+      for (int dtype = 0; dtype < 2; ++dtype) {
+        const char *kind = dtype ? "contract_f16" : "contract_bf16";
+        lth_pack::Group g;
+        g.dtype = dtype;
+        std::vector<int> M, N, K;
+        std::vector<void *> A16, Bt16, Cs;
+        for (int i = 0; i < n; ++i) {
+          const auto &pl = p->nests[i].h->plan();
+          const auto &mm = pl.matmul;
+          if (!mm.valid || pl.stages.empty() || pl.stages.back().kind != kind ||
+              mm.K % 8 || mm.M * mm.K * 2 >= (1LL << 31) || mm.N * mm.K * 2 >= (1LL << 31))
+            continue;
+          void *a = nullptr, *bt = nullptr;
+          chk(lt_hip_malloc(&a, mm.M * mm.K * 2), "pack operand");
+          p->staged.push_back(a);
+          chk(lt_hip_malloc(&bt, mm.N * mm.K * 2), "pack operand");
+          p->staged.push_back(bt);
+          const auto &nio = p->nests[i].io;
+          g.cvts.push_back({(const float *)nio[mm.a_input], a, (int)mm.M, (int)mm.K, 0});
+          g.cvts.push_back({(const float *)nio[mm.b_input], bt, (int)mm.K, (int)mm.N, 1});
+          M.push_back((int)mm.M);
+          N.push_back((int)mm.N);
+          K.push_back((int)mm.K);
+          A16.push_back(a);
+          Bt16.push_back(bt);
+          Cs.push_back(nio[p->nests[i].h->num_inputs()]);
+          g.est += p->nests[i].est;
+          grouped[i] = 1;
+        }
+        if (M.empty()) continue;
+        g.b = lth_batch_create((int)M.size(), M.data(), N.data(), K.data(), dtype);
+        if (!g.b) throw std::runtime_error(std::string("pack batch: ") + lth_batch_last_error());
+        p->groups.push_back(g);
+        if (lth_batch_set_operands(g.b, A16.data(), Bt16.data(), Cs.data()) != 0 ||
+            (p->groups.back().grid = lth_batch_prepare(g.b, 0)) < 0)
+          throw std::runtime_error(std::string("pack batch: ") + lth_batch_last_error());
+      }
+    }
+    // lanes: longest first, each nest (or group, numbered n + g) to the lane
+    // with the least estimated time
+    std::vector<int> order;
+    for (int i = 0; i < n; ++i)
+      if (!grouped[i]) order.push_back(i);
+    for (int g = 0; g < (int)p->groups.size(); ++g) order.push_back(n + g);
+    auto est = [&](int i) { return i < n ? p->nests[i].est : p->groups[i - n].est; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return est(a) > est(b); });
     std::vector<double> load(lanes, 0.0);
     std::vector<std::vector<int>> lane(lanes);
     for (int i : order) {
       const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       lane[l].push_back(i);
-      load[l] += p->nests[i].est;
+      load[l] += est(i);
     }
     for (int l = 1; l < lanes; ++l) {
       void *st = nullptr;
@@ -273,7 +338,19 @@ lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const 
       }
       for (int l = 0; l < lanes; ++l) {
         void *st = l == 0 ? stream : p->streams[l - 1];
-        for (int i : lane[l]) p->nests[i].h->run_on(p->nests[i].io, p->nests[i].scratch, st);
+        for (int i : lane[l]) {
+          if (i < n) {
+            p->nests[i].h->run_on(p->nests[i].io, p->nests[i].scratch, st);
+            continue;
+          }
+          const auto &g = p->groups[i - n];
+          for (const auto &c : g.cvts)
+            if (lth_convert(c.src, c.dst, c.rows, c.cols, c.transpose, g.dtype, st) != 0)
+              throw std::runtime_error(std::string("pack convert: ") + lth_batch_last_error());
+          if (lth_batch_run_grouped(g.b, g.grid, st) != 0)
+            throw std::runtime_error(std::string("pack grouped launch: ") +
+                                     lth_batch_last_error());
+        }
       }
       for (int l = 1; l < lanes; ++l) {
         chk(lt_hip_event_record(p->events[l], p->streams[l - 1]), "join");

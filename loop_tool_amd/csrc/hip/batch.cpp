@@ -427,6 +427,10 @@ int lth_batch_run_grouped(lth_batch *b, int grid, void *stream) {
   });
 }
 
+int lth_batch_prepare(lth_batch *b, int grid) {
+  return guarded([&] { return b->launch_grid(grid); });
+}
+
 int lth_batch_capture(lth_batch *b, int grid, void *stream) {
   return guarded([&] {
     ASSERT(stream) << "graph capture needs a created stream (not the null stream)";

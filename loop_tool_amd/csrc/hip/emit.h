@@ -39,6 +39,15 @@ struct Plan {
   // the configuration the lowering chose (tile_m, tile_n, bk, vec, block,
   // ...), exported as the Compiled's int_properties
   std::map<std::string, int64_t> config;
+  // set when the nest is a plain matmul: C[M][N] = sum_k A[M][K] * B[K][N]
+  // over inputs a_input (A) and b_input (B), all three row-major with no
+  // views, C the nest's only output (a pack can run such nests as one
+  // grouped GEMM launch)
+  struct Matmul {
+    bool valid = false;
+    int a_input = -1, b_input = -1;
+    int64_t M = 0, N = 0, K = 0;
+  } matmul;
 };
 
 // Knobs read from LoopTree annotations (see hip_backend.cpp).

@@ -195,6 +195,29 @@ bool try_contract(const Dataflow &df, const Knobs &knobs_in, Plan &plan) {
   for (auto &v : K) KD *= v.size;
   for (auto &v : B) BD *= v.size;
 
+  // a plain matmul: one var each for rows, columns and K, A[M][K] and
+  // B[K][N] read at exactly (i, k) and (k, j), C[M][N] row-major
+  {
+    auto plain = [&](const Operand &op, int s0, int s1, int64_t n0, int64_t n1) {
+      return op.sizes.size() == 2 && op.pred_dims.empty() && op.sizes[0] == n0 &&
+             op.sizes[1] == n1 && op.cst[0] == 0 && op.cst[1] == 0 &&
+             op.coef[0].size() == 1 && op.coef[0].count(s0) && op.coef[0].at(s0) == 1 &&
+             op.coef[1].size() == 1 && op.coef[1].count(s1) && op.coef[1].at(s1) == 1;
+    };
+    if (B.empty() && I.size() == 1 && J.size() == 1 && K.size() == 1 &&
+        W.vars.size() == 2 && df.sym(W.vars[0]).id() == I[0].sym &&
+        df.sym(W.vars[1]).id() == J[0].sym &&
+        plain(ops[po], I[0].sym, K[0].sym, MD, KD) && plain(ops[qo], K[0].sym, J[0].sym, KD, ND)) {
+      plan.matmul.valid = true;
+      plan.matmul.a_input = ops[po].input_index;
+      plan.matmul.b_input = ops[qo].input_index;
+      plan.matmul.M = MD;
+      plan.matmul.N = ND;
+      plan.matmul.K = KD;
+      plan.config["plain_matmul"] = 1;
+    }
+  }
+
   // staged operands and the implicit-GEMM tile (emit_igemm.cpp): 16-bit
   // operands, and float operands where the pipelined float tile applies
   if (!knobs.gather) {

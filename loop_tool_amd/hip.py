@@ -249,10 +249,11 @@ def time_stage_ms(compiled, io, stage, warmup=3, iters=20):
 
 
 _sig("lth_pack_create", _P, C.POINTER(_P), C.POINTER(_P), C.POINTER(C.c_int), C.c_int, C.c_int,
-     _P)
+     C.c_int, _P)
 _sig("lth_pack_replay", C.c_int, _P, _P)
 _L.lth_pack_free.restype = None
 _L.lth_pack_free.argtypes = [_P]
+PACK_GROUP_MATMULS = 1  # include/loop_tool_hip.h LTH_PACK_GROUP_MATMULS
 
 
 class Pack:
@@ -260,9 +261,11 @@ class Pack:
     lth_pack_*): `nests` is a list of (Compiled, [device buffers, inputs then
     outputs]); the nests are spread over `lanes` streams that run
     concurrently; `replay()` runs all of them with one graph launch on
-    `stream` (a created Stream's handle, by default the pack's own)."""
+    `stream` (a created Stream's handle, by default the pack's own).  With
+    `group_matmuls`, the plain 16-bit matmul nests run as one persistent
+    grouped GEMM launch per operand type."""
 
-    def __init__(self, nests, lanes=4, stream=None):
+    def __init__(self, nests, lanes=4, stream=None, group_matmuls=False):
         if stream is None:
             self._own_stream = Stream()
             stream = self._own_stream.h
@@ -271,7 +274,8 @@ class Pack:
         cc = (_P * len(nests))(*[c.h for c, _ in nests])
         flat = [b for _, io in nests for b in io]
         n_io = (C.c_int * len(nests))(*[len(io) for _, io in nests])
-        self.h = _L.lth_pack_create(cc, _ptrs(flat), n_io, len(nests), lanes, stream)
+        flags = PACK_GROUP_MATMULS if group_matmuls else 0
+        self.h = _L.lth_pack_create(cc, _ptrs(flat), n_io, len(nests), lanes, flags, stream)
         if not self.h:
             raise HipError(_L.lth_last_error().decode())
 

@@ -216,3 +216,17 @@ def test_build_script_runs_without_the_package():
     assert r.returncode == 0 and "--all" in r.stdout
     src = open(os.path.join(root, "loop_tool_amd", "build.py")).read()
     assert "import loop_tool_amd" not in src and "from loop_tool_amd" not in src
+
+
+def test_plain_matmul_descriptor(lt):
+    """Plans record when their nest is a plain row-major matmul (what a pack
+    may group into one batched GEMM launch); convolutions and batched
+    products are not."""
+    cases = [("matmul", dict(M=300, N=200, K=704), "hip_bf16", 1),
+             ("matmul", dict(M=128, N=128, K=128), "hip", 1),
+             ("conv2d", dict(N=2, Ci=32, Co=40, H=14, W=14), "hip_bf16", None),
+             ("bmm", dict(Bn=2, M=64, N=64, K=64), "hip_bf16", None)]
+    for name, kw, backend, want in cases:
+        out, _ = getattr(P, name)(lt, **kw)
+        cc = lt.compile(out.loop_tree, backend)
+        assert cc.int_property("plain_matmul") == want, (name, kw)
