@@ -78,6 +78,13 @@ def test_pack_refuses_bad_arguments(lt, hip_dev):
     cc, io = nests[0]
     with pytest.raises(Exception, match="device buffers"):
         hip_dev.Pack([(cc, io[:-1])], lanes=1)
+    # flags other than LTH_PACK_GROUP_MATMULS are refused
+    import ctypes as C
+    L = hip_dev._L
+    s = hip_dev.Stream()
+    h = L.lth_pack_create((C.c_void_p * 1)(cc.h), hip_dev._ptrs(io), (C.c_int * 1)(len(io)), 1, 1,
+                          2, s.h)
+    assert not h and b"unknown pack flags" in L.lth_last_error()
 
 
 def test_pack_runs_one_compiled_on_several_buffer_sets(lt, hip_dev):
