@@ -54,14 +54,22 @@ def _compile_worker(specs):
 
 def compile_nests(L, specs, workers=16):
     """[(Compiled, LoopTree, input shapes)] of the specs, compiled after a
-    parallel warm-up of the code cache."""
+    parallel warm-up of the code cache.  The warm-up is only a speed-up:
+    when its worker processes cannot start (a calling script without an
+    `if __name__ == "__main__":` guard, which spawned workers re-import), the
+    nests are compiled in this process instead."""
     import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from concurrent.futures.process import BrokenProcessPool
     uniq = sorted(set((n, tuple(sorted(kw.items())), b) for n, kw, b in specs))
     uniq = [(n, dict(kw), b) for n, kw, b in uniq]
     workers = max(1, min(workers, len(uniq)))
     if workers > 1:
-        with mp.get_context("spawn").Pool(workers) as pool:
-            pool.map(_compile_worker, [uniq[i::workers] for i in range(workers)])
+        try:
+            with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+                list(ex.map(_compile_worker, [uniq[i::workers] for i in range(workers)]))
+        except (BrokenProcessPool, RuntimeError):
+            pass
     out = []
     for name, kw, backend in specs:
         prog, shapes = getattr(P, name)(L, **kw)
