@@ -305,13 +305,20 @@ def prewarm_code_cache(dims, dtype="bf16", workers=16):
     """Compile the matmul nests of `dims` in worker processes into the hip
     backend's on-disk code cache (hipRTC compiles of one process run one at
     a time; processes run in parallel), so that compiling them here only
-    loads cached code objects."""
+    loads cached code objects.  Returns the number compiled there: 0 when
+    the workers cannot start (a calling script without an
+    `if __name__ == "__main__":` guard), and then they compile on first use."""
     import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from concurrent.futures.process import BrokenProcessPool
     dims = sorted(set(tuple(d) for d in dims))
     workers = max(1, min(workers, len(dims)))
-    with mp.get_context("spawn").Pool(workers) as pool:
-        return sum(pool.map(_compile_nests, [(dims[i::workers], dtype)
-                                             for i in range(workers)]))
+    try:
+        with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+            return sum(ex.map(_compile_nests, [(dims[i::workers], dtype)
+                                               for i in range(workers)]))
+    except (BrokenProcessPool, RuntimeError):
+        return 0
 
 
 class PlainNests:
