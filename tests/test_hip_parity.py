@@ -478,16 +478,21 @@ def test_run_without_sync_on_device_buffers(lt, hip_dev):
     assert np.array_equal(yh.view(np.uint32), y.view(np.uint32))
 
 
-def test_runs_from_threads_run_apart(lt, hip_dev):
+@pytest.mark.parametrize("prog", [("conv2d", (4, 64, 64, 28, 28), 64 * 9),
+                                  ("matmul", (368, 1600, 1984), 1984)],
+                         ids=["conv-halo", "matmul-direct-lds"])
+def test_runs_from_threads_run_apart(lt, hip_dev, prog):
     """Compiled::run called from several host threads at once: each thread
     launches on its own stream with its own scratch and device copies of
     host buffers, so one Compiled serves them concurrently (the reference's
     cpp backend is not re-entrant, cpp.cpp:454-455).  Every thread's results
     equal the single-threaded run bit for bit, for host buffers (sync) and
-    for device buffers queued with sync=false and then synchronised."""
+    for device buffers queued with sync=false and then synchronised.  The
+    programs: a conv on the halo tile and a matmul on the direct-to-LDS tile,
+    whose kernels then share CUs."""
     import threading
-    out, shapes = P.conv2d(lt, 4, 64, 64, 28, 28)
-    fan = 64 * 9
+    name, args, fan = prog
+    out, shapes = getattr(P, name)(lt, *args)
     sets = [[P.normal(60 + 2 * i, shapes[0], fan ** -0.5),
              P.normal(61 + 2 * i, shapes[1], fan ** -0.5)] for i in range(4)]
     t = out.loop_tree
