@@ -252,11 +252,6 @@ lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const 
       nest.io.assign(io + off, io + off + n_io[i]);
       off += n_io[i];
       nest.h->prepare();  // code objects load outside the capture
-      for (auto words : nest.h->scratch_sizes()) {
-        void *b = nullptr;
-        chk(lt_hip_malloc(&b, std::max<int64_t>(words, 1) * sizeof(float)), "pack scratch");
-        nest.scratch.push_back(b);
-      }
       const bool f32 = nest.h->plan().stages.back().kind == "contract_f32";
       nest.est = std::max(nest.h->algorithmic_bytes() / 8e12,
                           nest.h->algorithmic_flops() / (f32 ? 157.3e12 : 2.5e15));
@@ -303,6 +298,15 @@ lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const 
         if (lth_batch_set_operands(g.b, A16.data(), Bt16.data(), Cs.data()) != 0 ||
             (p->groups.back().grid = lth_batch_prepare(g.b, 0)) < 0)
           throw std::runtime_error(std::string("pack batch: ") + lth_batch_last_error());
+      }
+    }
+    // scratch of the nests that run their own plans
+    for (int i = 0; i < n; ++i) {
+      if (grouped[i]) continue;
+      for (auto words : p->nests[i].h->scratch_sizes()) {
+        void *b = nullptr;
+        chk(lt_hip_malloc(&b, std::max<int64_t>(words, 1) * sizeof(float)), "pack scratch");
+        p->nests[i].scratch.push_back(b);
       }
     }
     // lanes: longest first, each nest (or group, numbered n + g) to the lane
