@@ -107,10 +107,13 @@ int lth_pack_replay(lth_pack *p, void *stream);
 void lth_pack_free(lth_pack *p);
 
 /* ---- batches of run-time-sized matmul nests (C = A B, 16-bit operands,
- * float32 accumulation and output).  Operands live in HBM as A[M][K] and the
- * transposed Bt[N][K]; tile configurations 0..6 are 64x64x32, 128x64x32,
- * 64x128x32, 128x128x64, 256x128x32, 128x256x32 and 128x128x32 (rows x cols
- * of C x K tile). */
+ * float32 accumulation and output).  Operands live in HBM as A[M][ld] and the
+ * transposed Bt[N][ld], ld = LTH_BATCH_PITCH(K) (K rounded up to a multiple
+ * of 64, so every row starts on a 128-byte line; columns K..ld-1 are never
+ * read), as lth_convert writes them; C is [M][N].  Tile configurations 0..6
+ * are 64x64x32, 128x64x32, 64x128x32, 128x128x64, 256x128x32, 128x256x32 and
+ * 128x128x32 (rows x cols of C x K tile). */
+#define LTH_BATCH_PITCH(k) (((k) + 63) / 64 * 64)
 typedef struct lth_batch lth_batch;
 const char *lth_batch_last_error(void);
 /* dtype: 0 bf16, 1 fp16; K must be a multiple of 8 */
@@ -136,8 +139,9 @@ int lth_batch_prepare(lth_batch *b, int grid);
 int lth_batch_capture(lth_batch *b, int grid, void *stream);
 int lth_batch_replay(lth_batch *b, void *stream);
 void lth_batch_free(lth_batch *b);
-/* float32 [rows][cols] -> 16-bit operand (transposed to [cols][rows] if
- * transpose != 0), on `stream` */
+/* float32 [rows][cols] -> 16-bit operand [rows][LTH_BATCH_PITCH(cols)], or
+ * transposed to [cols][LTH_BATCH_PITCH(rows)] if transpose != 0 (the padding
+ * is not written), on `stream` */
 int lth_convert(const float *src, void *dst, int rows, int cols, int transpose, int dtype,
                 void *stream);
 int lth_tile_config(int cfg, int *bm, int *bn);

@@ -51,6 +51,12 @@ def _chk(rc):
     return rc
 
 
+def pitch(k):
+    """Row pitch (elements) of the batch's 16-bit operands: K rounded up to
+    a multiple of 64 (include/loop_tool_hip.h, LTH_BATCH_PITCH)."""
+    return (k + 63) // 64 * 64
+
+
 def shapes(n=512, seed=6, lo=32, hi=2048, mult=16):
     u = P.uniform(seed, (n, 3)).astype(np.float64) * 0.5 + 0.5  # [0, 1)
     v = np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))
@@ -83,7 +89,8 @@ class Batch:
             b = P.uniform(seed * 1000003 + 2 * i + 1, (k, nn))
             self.host.append((a, b))
             a_d, b_d = DeviceArray.from_numpy(a), DeviceArray.from_numpy(b)
-            A16, Bt16 = DeviceArray(m * k // 2 + 1), DeviceArray(nn * k // 2 + 1)
+            kp = pitch(k)  # rows padded to whole 128-byte lines
+            A16, Bt16 = DeviceArray(m * kp // 2 + 1), DeviceArray(nn * kp // 2 + 1)
             _chk(_L.lth_convert(a_d.ptr, A16.ptr, m, k, 0, self.dtype, self.stream))
             _chk(_L.lth_convert(b_d.ptr, Bt16.ptr, k, nn, 1, self.dtype, self.stream))
             _L.lt_hip_stream_synchronize(self.stream)  # before a_d, b_d are freed

@@ -271,13 +271,14 @@ lth_pack *lth_pack_create(const ltc_compiled *const *cc, void *const *io, const 
         for (int i = 0; i < n; ++i) {
           const auto &pl = p->nests[i].h->plan();
           const auto &mm = pl.matmul;
+          const int64_t ld = LTH_BATCH_PITCH(mm.K);
           if (!mm.valid || pl.stages.empty() || pl.stages.back().kind != kind ||
-              mm.K % 8 || mm.M * mm.K * 2 >= (1LL << 31) || mm.N * mm.K * 2 >= (1LL << 31))
+              mm.K % 8 || mm.M * ld * 2 >= (1LL << 31) || mm.N * ld * 2 >= (1LL << 31))
             continue;
           void *a = nullptr, *bt = nullptr;
-          chk(lt_hip_malloc(&a, mm.M * mm.K * 2), "pack operand");
+          chk(lt_hip_malloc(&a, mm.M * ld * 2), "pack operand");
           p->staged.push_back(a);
-          chk(lt_hip_malloc(&bt, mm.N * mm.K * 2), "pack operand");
+          chk(lt_hip_malloc(&bt, mm.N * ld * 2), "pack operand");
           p->staged.push_back(bt);
           const auto &nio = p->nests[i].io;
           g.cvts.push_back({(const float *)nio[mm.a_input], a, (int)mm.M, (int)mm.K, 0});

@@ -369,9 +369,10 @@ lth_batch *lth_batch_create(int n, const int *M, const int *N, const int *K, int
           << "GEMM " << i << ": sizes must be positive and K a multiple of 8";
       // the tile addresses operands through 32-bit buffer ranges and row
       // offsets: each 16-bit operand must stay under 2^31 bytes
-      ASSERT((int64_t)M[i] * K[i] * 2 < (1LL << 31) && (int64_t)N[i] * K[i] * 2 < (1LL << 31))
-          << "GEMM " << i << ": a 16-bit operand of " << M[i] << "x" << K[i] << " or "
-          << N[i] << "x" << K[i] << " reaches 2^31 bytes";
+      const int64_t ld = LTH_BATCH_PITCH((int64_t)K[i]);
+      ASSERT((int64_t)M[i] * ld * 2 < (1LL << 31) && (int64_t)N[i] * ld * 2 < (1LL << 31))
+          << "GEMM " << i << ": a 16-bit operand of " << M[i] << "x" << ld << " or "
+          << N[i] << "x" << ld << " reaches 2^31 bytes";
       b->descs.push_back({nullptr, nullptr, nullptr, M[i], N[i], K[i], 0});
     }
     b->cfg.assign(n, 3);

@@ -7,7 +7,8 @@
 //
 // Every launch form runs the direct-to-LDS pipelined implicit-GEMM tile of
 // igemm_tile.h (with its K-tail masking: K is any multiple of 8) on operands
-// A[M][K] and Bt[N][K] (already K-innermost, so no staging).  The grouped
+// A[M][ld] and Bt[N][ld], ld = lt_mm_ld(K) (already K-innermost, so no
+// staging).  The grouped
 // kernels walk a host-built work list of output tiles from many GEMMs, cut
 // into one segment per XCD and ordered by decreasing cost within each, so
 // that the longest tiles start first, the workgroups finish together, and
@@ -36,17 +37,24 @@ struct LtWorkItem {
   int M, N, K, m0, n0, cfg;
 };
 
-// geometry of a plain matmul for the tile: rows of A and Bt are K apart
+// Row pitch (elements) of the 16-bit operands A[M][pitch] and Bt[N][pitch]:
+// K rounded up to 64, so that every row starts on a 128-byte line (with K a
+// multiple of 16 only, a K tile's rows would straddle two lines each); the
+// columns past K are never read
+__host__ __device__ constexpr int lt_mm_ld(int K) { return (K + 63) / 64 * 64; }
+
+// geometry of a plain matmul for the tile: rows of A and Bt are lt_mm_ld(K)
+// apart
 struct LtMmGeo {
   int KO, KI;
-  long MD, ND, N;
+  long MD, ND, N, LD;
   __device__ __forceinline__ long p_row(long i, bool& ok) const {
     ok = i < MD;
-    return i * KI;
+    return i * LD;
   }
   __device__ __forceinline__ long q_row(long j, bool& ok) const {
     ok = j < ND;
-    return j * KI;
+    return j * LD;
   }
   __device__ __forceinline__ long p_koff(int) const { return 0; }
   __device__ __forceinline__ long q_koff(int) const { return 0; }
@@ -61,6 +69,7 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
   g.MD = M;
   g.ND = N;
   g.N = N;
+  g.LD = lt_mm_ld(K);
   return g;
 }
 
@@ -84,14 +93,14 @@ __device__ __forceinline__ LtMmGeo lt_mm_geo(int M, int N, int K) {
     __shared__ __attribute__((aligned(16))) unsigned char smem[lt_igemm_dl_lds_bytes<BM, BN, BK>()]; \
     lt_igemm_tile_dl<T, BM, BN, BK, false, 2, 2, true>(                                     \
         lt_mm_geo(M, N, K), A, Bt, C, (long)blockIdx.y * BM, (long)blockIdx.x * BN, smem,  \
-        (unsigned)M * K * 2, (unsigned)N * K * 2);                                          \
+        (unsigned)M * lt_mm_ld(K) * 2, (unsigned)N * lt_mm_ld(K) * 2);                      \
   }
 
 #define LT_CASE(i, BM, BN, BK)                                                               \
   case i:                                                                                    \
     lt_igemm_tile_dl<T, BM, BN, BK, false, 2, 2, true>(                                     \
-        g, A, Bt, it.C, it.m0, it.n0, smem, (unsigned)it.M * it.K * 2,                      \
-        (unsigned)it.N * it.K * 2);                                                         \
+        g, A, Bt, it.C, it.m0, it.n0, smem, (unsigned)it.M * lt_mm_ld(it.K) * 2,            \
+        (unsigned)it.N * lt_mm_ld(it.K) * 2);                                               \
     break;
 
 // Persistent loop of a batch whose items all use one tile configuration:
@@ -145,19 +154,20 @@ __device__ __forceinline__ void lt_ggemm_static(const LtWorkItem* __restrict__ i
   };
   auto job = [&](const LtWorkItem& it) {
     Job j;
+    const unsigned ld = (unsigned)lt_mm_ld(it.K);
     j.rP = __builtin_amdgcn_make_buffer_rsrc((void*)it.A, (short)0,
-                                             (int)((unsigned)it.M * it.K * 2), 0x00020000);
+                                             (int)((unsigned)it.M * ld * 2), 0x00020000);
     j.rQ = __builtin_amdgcn_make_buffer_rsrc((void*)it.Bt, (short)0,
-                                             (int)((unsigned)it.N * it.K * 2), 0x00020000);
+                                             (int)((unsigned)it.N * ld * 2), 0x00020000);
 #pragma unroll
     for (int v = 0; v < VP; ++v) {
       const int row = it.m0 + pr[v];
-      j.pv[v] = row < it.M ? ((unsigned)row * it.K + pk[v]) * 2 : 0x80000000u;
+      j.pv[v] = row < it.M ? ((unsigned)row * ld + pk[v]) * 2 : 0x80000000u;
     }
 #pragma unroll
     for (int v = 0; v < VQ; ++v) {
       const int row = it.n0 + qr[v];
-      j.qv[v] = row < it.N ? ((unsigned)row * it.K + qk[v]) * 2 : 0x80000000u;
+      j.qv[v] = row < it.N ? ((unsigned)row * ld + qk[v]) * 2 : 0x80000000u;
     }
     j.K = it.K;
     j.KT = (it.K + BK - 1) / BK;
@@ -324,7 +334,9 @@ LT_PLAIN_ALL(_Float16, f16)
 LT_GROUPED_ALL(__bf16, bf16)
 LT_GROUPED_ALL(_Float16, f16)
 
-// float32 [rows][cols] -> 16-bit [rows][cols], or [cols][rows] when transpose
+// float32 [rows][cols] -> 16-bit [rows][lt_mm_ld(cols)], or
+// [cols][lt_mm_ld(rows)] when transpose (the operand layout of the tiles
+// above; the padding columns are not written)
 #define LT_CVT(T, TN)                                                                       \
   extern "C" __global__ void __launch_bounds__(256)                                         \
       lt_cvt_##TN(const float* __restrict__ src, T* __restrict__ dst, int rows, int cols,   \
@@ -334,9 +346,9 @@ LT_GROUPED_ALL(_Float16, f16)
     const int r = (int)(i / cols), c = (int)(i % cols);                                     \
     const T v = (T)src[i];                                                                  \
     if (transpose)                                                                          \
-      dst[(long)c * rows + r] = v;                                                          \
+      dst[(long)c * lt_mm_ld(rows) + r] = v;                                                \
     else                                                                                    \
-      dst[i] = v;                                                                           \
+      dst[(long)r * lt_mm_ld(cols) + c] = v;                                                \
   }
 
 LT_CVT(__bf16, bf16)
