@@ -194,7 +194,13 @@ int64_t staging_plane_body(std::stringstream &k, const std::string &name, const 
   int64_t RA = rows > 0 ? rows : std::max<int64_t>(1, 256 / L);
   RA = std::max<int64_t>(1, std::min<int64_t>({RA, s.ssize[a], 65536 / (L * TP * (wide ? 4 : 2))}));
   const int64_t NP = RA * L, NQ = (NP + 3) / 4, NQP = (NQ + 7) / 8 * 8, CP = TI / 2;
-  const int64_t nA = (s.ssize[a] + RA - 1) / RA, nC = s.ssize[in] / TI;
+  // row blocks start at source rows that are multiples of RA (the staged
+  // rows shifted by the halo), so that a block's RA source rows start where
+  // a block of rows starts in the source and their runs begin on the same
+  // cache-line boundaries (a block may begin with halo rows below 0, which
+  // are skipped)
+  const int64_t ashift = (RA - s.padlo[a] % RA) % RA;
+  const int64_t nA = (s.ssize[a] + ashift + RA - 1) / RA, nC = s.ssize[in] / TI;
   std::vector<int> other;
   for (int q = 0; q < nd; ++q)
     if (s.order[q] != ld && s.order[q] != in && s.order[q] != a) other.push_back(s.order[q]);
@@ -212,7 +218,8 @@ int64_t staging_plane_body(std::stringstream &k, const std::string &name, const 
     << "  __shared__ __attribute__((aligned(16))) " << elt << " tile[" << NP * TP << "];\n"
     << "  const int tid = threadIdx.x;\n"
     << "  const long c0 = (long)(bid % " << nC << "u) * " << TI << ";\n"
-    << "  const long a0 = (long)((bid / " << nC << "u) % " << nA << "u) * " << RA << ";\n"
+    << "  const long a0 = (long)((bid / " << nC << "u) % " << nA << "u) * " << RA << " - "
+    << ashift << "L;\n"
     << "  long z_ = bid / " << nC * nA << "u;\n";
   std::string ok, src, dst;
   fixed_dims(k, op, s, other, ok, src, dst);
@@ -260,6 +267,7 @@ int64_t staging_plane_body(std::stringstream &k, const std::string &name, const 
     << "    const long sl = (e / " << TI / 8 << ") % " << s.ssize[ld] << ", r = e / "
     << (TI / 8) * s.ssize[ld] << ";\n"
     << "    if (a0 + r >= " << s.ssize[a] << "L) break;\n"
+    << "    if (a0 + r < 0L) continue;\n"
     << "    const long w = sl - " << pl << "L, ra = a0 + r - " << pa << "L;\n"
     << "    T8 v = {};\n"
     << "    if (fixed_ok && w >= 0 && w < " << L << "L && ra >= 0 && ra < " << op.sizes[a]
